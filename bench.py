@@ -1,0 +1,186 @@
+"""Benchmark of the extended-LBM MHD time step on one MI355X.
+
+A step is one full time step of the reference's hot path (Lbm::do_time_step,
+src/lbm/mod.rs:250-272): clear_qu_lod, stream_collide (fluid + electron gas +
+D3Q7 charge advection + Lorentz force + LOD sums), the LOD tree and the
+update_e_b_dynamic field step, over the whole lattice.
+
+Default workload (the configuration the headline metric is quoted on):
+256^3 D3Q19 SRT, FP32 DDFs, MHD with 8 seeded Gaussian charge blobs
+(sigma = 6 cells), uniform B0 plus a point dipole, Guo Lorentz forcing, a
+solid cylinder r = 40, LOD depth 4. Inputs are resident in HBM before the
+timed region. Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MHD-LBM MLUPs/s + %HBM peak; surrogate vs direct field-step speedup @≤1% err"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FLOP_PER_INTERACTION = 35  # r 3, r^2 5, sqrt 1, cube 2, r/|r|^3 3, e 6, v x pre 9, b 6
+
+
+def bytes_per_cell(cfg):
+    """Algorithmic HBM bytes of stream_collide per non-solid cell (each DDF
+    read once and written once, fields as the kernel touches them)."""
+    q = cfg.velocity_set.get_set_values()[1]
+    s = cfg.float_type.size_of()
+    b = 2 * q * s + 1  # fi read + write, flags
+    if cfg.ext_magneto_hydro:
+        b += 2 * q * s + 2 * 7 * s + 24 + 4  # ei, fqi, E_dyn+B_dyn read, Q write
+    if cfg.graphics_config.graphics_active:
+        b += 16  # rho, u written (UPDATE_FIELDS)
+    return b
+
+
+def workload(name):
+    from ionsolver_amd import FloatType, scenarios
+    ft = {"fp32": FloatType.FP32, "fp16s": FloatType.FP16S, "fp16c": FloatType.FP16C}
+    if name.startswith("mhd"):
+        parts = name.split("-")
+        n = int(parts[0][3:])
+        fmt = parts[1] if len(parts) > 1 else "fp32"
+        cfg, arrays, blobs = scenarios.mhd_blobs(n=n, float_type=ft[fmt])
+        return cfg, arrays, dict(blobs=len(blobs), seed=1234)
+    if name.startswith("tgv"):
+        parts = name.split("-")
+        n = int(parts[0][3:])
+        fmt = parts[1] if len(parts) > 1 else "fp32"
+        cfg, arrays = scenarios.fluid_tgv(n=n, float_type=ft[fmt])
+        return cfg, arrays, dict(u0=0.05, tau=0.6)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(name, seconds_hint):
+    """The oracle (CPU restatement of the reference kernels, OpenMP over all
+    host cores) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from harness import oracle_for
+    from ionsolver_amd import scenarios
+    if name.startswith("mhd"):
+        n_sample = 128 if int(name.split("-")[0][3:]) >= 128 else int(name.split("-")[0][3:])
+        cfg, arrays, _ = scenarios.mhd_blobs(n=n_sample, float_type=workload(name)[0].float_type)
+        steps = 1
+    else:
+        n_sample = int(name.split("-")[0][3:])
+        cfg, arrays = scenarios.fluid_tgv(n=n_sample, float_type=workload(name)[0].float_type)
+        steps = 50
+    orc = oracle_for(cfg)
+    for k, v in arrays.items():
+        orc.buf(k)[:] = v
+    orc.initialize()
+    if not cfg.ext_magneto_hydro:
+        orc.do_time_step()  # warm-up (an MHD step is long enough on its own)
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps or time.perf_counter() - t0 < seconds_hint:
+        orc.do_time_step()
+        done += 1
+        if time.perf_counter() - t0 > 3 * seconds_hint:
+            break
+    dt = time.perf_counter() - t0
+    cells = cfg.n_x * cfg.n_y * cfg.n_z
+    return {"value": round(cells * done / dt / 1e6, 4), "unit": "MLUPs/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"{done} step(s) of the same workload at {n_sample}^3"
+                      + (" (same LOD depth, so the same field-update sources per cell)" if cfg.ext_magneto_hydro else "")
+                      + ", oracle built -O3 -fopenmp"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="mhd256")
+    ap.add_argument("--field-mode", default="exact", choices=["exact", "fast"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import ionsolver_amd as I
+    from ionsolver_amd import FieldMode, scenarios
+
+    cfg, arrays, meta = workload(args.workload)
+    lbm = I.Lbm(cfg)
+    scenarios.apply(lbm, arrays)
+    lbm.set_field_mode(FieldMode.FAST if args.field_mode == "fast" else FieldMode.EXACT)
+    lbm.initialize()
+    if args.warmup:
+        lbm.run(args.warmup)
+    lbm.set_timing(True)
+    lbm.synchronize()
+    t0 = time.perf_counter()
+    lbm.run(args.steps)  # enqueues all steps, synchronises at the end
+    dt = time.perf_counter() - t0
+    tm = lbm.get_timing()
+    lbm.set_timing(False)
+
+    N = cfg.n_x * cfg.n_y * cfg.n_z
+    flags = arrays.get("flags")
+    n_f = int(np.sum((flags & 0x1F) != 1)) if flags is not None else N
+    mlups = N * args.steps / dt / 1e6
+    k = args.steps
+    sc_ms = tm["ms_stream_collide"] / k
+    sc_bytes = n_f * bytes_per_cell(cfg) + (N - n_f) * 1
+    kernels = {"stream_collide": {"ms": round(sc_ms, 4), "bytes": sc_bytes,
+                                  "GB/s": round(sc_bytes / sc_ms / 1e6, 1) if sc_ms else None,
+                                  "frac_hbm": round(sc_bytes / sc_ms / 1e6 / HBM_PEAK_GBS, 4) if sc_ms else None}}
+    roofline = None
+    if cfg.ext_magneto_hydro:
+        D = cfg.mhd_lod_depth
+        nd = 1 << D
+        n_top = nd ** 3
+        lo = ((1 << (3 * (D + 1))) - 1) // 7 - n_top
+        # own LOD index of every fluid cell; interactions = N_f*(sources in range) - #(own LOD in range)
+        z, rem = np.divmod(np.arange(N, dtype=np.int64), cfg.n_x * cfg.n_y)
+        y, x = np.divmod(rem, cfg.n_x)
+        own = x // (cfg.n_x // nd) + (y // (cfg.n_y // nd) + z // (cfg.n_z // nd) * nd) * nd
+        fl = (flags & 0x1F) != 1
+        in_range = n_top - lo  # top-level LODs inside [lo, n_lod_own)
+        interactions = n_f * in_range - int(np.sum(fl & (own >= lo)))
+        eb_ms = tm["ms_update_e_b"] / k
+        eb_tflops = interactions * FLOP_PER_INTERACTION / (eb_ms * 1e-3) / 1e12
+        kernels["update_e_b_dynamic"] = {"ms": round(eb_ms, 4), "interactions": interactions,
+                                         "TFLOP/s": round(eb_tflops, 2)}
+        kernels["lod_reduce"] = {"ms": round(tm["ms_lod_reduce"] / k, 4)}
+        kernels["field_prep"] = {"ms": round(tm["ms_field_prep"] / k, 4)}
+        kernels["clear_qu_lod"] = {"ms": round(tm["ms_other"] / k, 4)}
+        roofline = {"bound": "mfma", "achieved": round(eb_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(eb_tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                    "kernel": "update_e_b_dynamic", "compute": "fp32 VALU"}
+    else:
+        roofline = {"bound": "hbm", "achieved": kernels["stream_collide"]["GB/s"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": kernels["stream_collide"]["frac_hbm"], "traffic": None,
+                    "kernel": "stream_collide"}
+    out = {
+        "metric": METRIC, "value": round(mlups, 3), "unit": "MLUPs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": {0: "fp16s storage / f32", 1: "fp16c storage / f32", 2: "f32"}[int(cfg.float_type)],
+        "data": "synthetic", "config": {"workload": args.workload, "lattice": [cfg.n_x, cfg.n_y, cfg.n_z],
+                                        "velocity_set": cfg.velocity_set.name, "float_type": cfg.float_type.name,
+                                        "mhd": bool(cfg.ext_magneto_hydro), "lod_depth": cfg.mhd_lod_depth,
+                                        "field_mode": args.field_mode, "fluid_cells": n_f, **meta},
+        "roofline": roofline, "kernels": kernels,
+    }
+    lbm.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,191 @@
+/*
+ * ionsolver.h - C ABI of the MI355X (gfx950) backend of the IonSolver
+ * extended-LBM MHD time step (libionsolver_hip.so).
+ *
+ * The ABI replaces the reference's Rust `Lbm`/`LbmConfig` API over its
+ * OpenCL `LbmDomain` (src/lbm/mod.rs, src/lbm/domain.rs, src/opencl.rs)
+ * for a single GPU. Every entry point cites the reference interface it
+ * replaces. Conventions:
+ *   - every function returns ION_OK (0) or a negative ion_status, and never
+ *     aborts; ion_last_error() gives the message of the last failure on the
+ *     calling thread;
+ *   - one handle owns all of its device memory and one HIP stream; calls on
+ *     one handle must be serialised by the caller;
+ *   - host pointers are borrowed for the duration of a call; field copies are
+ *     synchronous, like the reference's blocking bread!/bwrite!.
+ */
+#ifndef IONSOLVER_H
+#define IONSOLVER_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ion_status {
+    ION_OK = 0,
+    ION_ERR_INVALID = -1,     /* invalid argument or configuration */
+    ION_ERR_UNSUPPORTED = -2, /* valid for the reference, not for this backend */
+    ION_ERR_DEVICE = -3,      /* HIP runtime error */
+    ION_ERR_IO = -4,          /* file error */
+    ION_ERR_FORMAT = -5       /* malformed save file */
+} ion_status;
+
+/* VelocitySet, src/lbm/types.rs:16-26 (same discriminants) */
+typedef enum { ION_D2Q9 = 0, ION_D3Q15 = 1, ION_D3Q19 = 2, ION_D3Q27 = 3 } ion_velocity_set;
+/* RelaxationTime, src/lbm/types.rs:55-61 */
+typedef enum { ION_SRT = 0, ION_TRT = 1 } ion_relaxation_time;
+/* FloatType, src/lbm/types.rs:75-83 */
+typedef enum { ION_FP16S = 0, ION_FP16C = 1, ION_FP32 = 2 } ion_float_type;
+/* Propellant, src/lbm/units.rs:214-222 */
+typedef enum { ION_PROP_H = 0, ION_PROP_HE, ION_PROP_NE, ION_PROP_AR, ION_PROP_KR, ION_PROP_XE } ion_propellant;
+
+/* Units, src/lbm/units.rs:13-26: lattice unit * factor = SI unit. */
+typedef struct ion_units {
+    float m, kg, s, a, k;
+    int32_t prop;
+} ion_units;
+
+/* LbmConfig, src/lbm/mod.rs:45-101. graphics_active stands for
+ * graphics_config.graphics_active, the only GraphicsConfig field that reaches
+ * the kernels (it switches UPDATE_FIELDS on, src/lbm/domain.rs:856). */
+typedef struct ion_config {
+    int32_t velocity_set;
+    int32_t relaxation_time;
+    int32_t float_type;
+    ion_units units;
+    uint32_t n_x, n_y, n_z;
+    uint32_t d_x, d_y, d_z;
+    float nu;
+    float f_x, f_y, f_z;
+    uint8_t ext_equilibrium_boudaries;
+    uint8_t ext_volume_force;
+    uint8_t ext_force_field;
+    uint8_t ext_magneto_hydro;
+    uint8_t ext_subgrid_ecr;
+    uint8_t mhd_lod_depth;
+    uint8_t graphics_active;
+    uint8_t reserved0;
+    float ecr_freq;
+    float ecr_field_strength;
+    uint64_t run_steps;
+} ion_config;
+
+/* Device buffers of one domain (src/lbm/domain.rs:45-74), all SoA:
+ * scalar fields [N] f32, vector fields [3][N] f32, flags [N] u8,
+ * DDFs [q][N] (f32 for FP32, u16 codes for FP16S/FP16C), QU_lod [n_lod][4] f32. */
+typedef enum ion_field {
+    ION_FIELD_FI = 0,
+    ION_FIELD_RHO = 1,
+    ION_FIELD_U = 2,
+    ION_FIELD_FLAGS = 3,
+    ION_FIELD_F = 4,       /* force field (ext_force_field) */
+    ION_FIELD_E_STAT = 5,
+    ION_FIELD_B_STAT = 6,
+    ION_FIELD_E_DYN = 7,
+    ION_FIELD_B_DYN = 8,
+    ION_FIELD_FQI = 9,     /* D3Q7 charge DDFs */
+    ION_FIELD_Q = 10,
+    ION_FIELD_QU_LOD = 11,
+    ION_FIELD_EI = 12,     /* electron-gas DDFs */
+    ION_FIELD_COUNT = 13
+} ion_field;
+
+/* The f32 kernel constants the reference bakes in as #defines
+ * (src/lbm/domain.rs:808, 836-849), derived on the host in f32 in the
+ * order of src/lbm/units.rs. */
+typedef struct ion_kernel_constants {
+    float w;     /* DEF_W  = 1/(3 nu + 1/2) */
+    float c;     /* DEF_C  = 0.57735027 */
+    float ke;    /* DEF_KE = 1/(4 pi eps0_lu) */
+    float kmu;   /* DEF_KMU = mu0_lu/(4 pi) */
+    float kmu0;  /* DEF_KMU0 = mu0_lu */
+    float kkge;  /* DEF_KKGE = electron mass/charge in lattice units */
+    float wq;    /* DEF_WQ = 1/(2 k + 1/2), k = 1 */
+    uint32_t n_lod;     /* DEF_NUM_LOD (single domain: = n_lod_own) */
+} ion_kernel_constants;
+
+/* Field-update arithmetic (build-side option, no reference counterpart):
+ *   ION_FIELD_EXACT: the reference's r/|r|^3 with correctly rounded sqrt and
+ *                    division and unfused accumulation (default);
+ *   ION_FIELD_FAST:  hardware reciprocal square root and fused accumulation,
+ *                    within a stated relative tolerance of EXACT. */
+typedef enum { ION_FIELD_EXACT = 0, ION_FIELD_FAST = 1 } ion_field_mode;
+
+/* Accumulated GPU time per kernel (hipEvent pairs on the handle's stream),
+ * enabled by ion_set_timing. */
+typedef struct ion_timing {
+    double ms_stream_collide;
+    double ms_lod_reduce;
+    double ms_field_prep;
+    double ms_update_e_b;
+    double ms_other;
+    uint64_t n_steps;
+} ion_timing;
+
+typedef struct ion_handle ion_handle;
+
+/* LbmConfig::new, src/lbm/mod.rs:105-135 (graphics_active default true,
+ * src/lbm/graphics.rs:160). */
+void ion_config_default(ion_config* cfg);
+/* Units::new / Units::set, src/lbm/units.rs:30-59. */
+void ion_units_default(ion_units* u);
+void ion_units_set(ion_units* u, float lbm_length, float lbm_velocity, float lbm_rho, float lbm_charge,
+                   float lbm_temp, float si_length, float si_velocity, float si_rho, float si_charge, float si_temp);
+/* get_device_defines constants, src/lbm/domain.rs:736-857, units.rs:149-177. */
+int ion_kernel_constants_of(const ion_config* cfg, ion_kernel_constants* out);
+
+/* Lbm::new, src/lbm/mod.rs:166-212 (single domain: d_x = d_y = d_z = 1).
+ * Allocates and zero-fills the buffers as LbmDomain::new does
+ * (src/lbm/domain.rs:151-211: rho = 1, everything else 0). */
+int ion_create(const ion_config* cfg, ion_handle** out);
+void ion_destroy(ion_handle* h);
+int ion_get_config(const ion_handle* h, ion_config* out);
+
+/* Lbm::initialize, src/lbm/mod.rs:214-231. */
+int ion_initialize(ion_handle* h);
+/* Lbm::run, src/lbm/mod.rs:235-246 (initialises lazily). */
+int ion_run(ion_handle* h, uint64_t steps);
+/* Lbm::do_time_step, src/lbm/mod.rs:250-272. */
+int ion_do_time_step(ion_handle* h);
+/* Lbm::get_time_step, src/lbm/mod.rs:492-494. */
+uint64_t ion_get_time_step(const ion_handle* h);
+/* LbmDomain::enqueue_update_fields, src/lbm/domain.rs:430-441. */
+int ion_update_fields(ion_handle* h);
+/* Blocks until the handle's stream is idle (Lbm::finish_queues, mod.rs:275-279). */
+int ion_synchronize(ion_handle* h);
+
+/* bwrite!/bread! on the domain buffers (e.g. src/setup.rs:302, src/file.rs:233).
+ * `bytes` must equal the buffer size reported by ion_field_size. */
+int ion_field_size(const ion_handle* h, int field, size_t* bytes, uint32_t* elem_size);
+int ion_write_field(ion_handle* h, int field, const void* src, size_t bytes);
+int ion_read_field(ion_handle* h, int field, void* dst, size_t bytes);
+/* Volume force kernel arguments, src/lbm/domain.rs:422-425. */
+int ion_set_volume_force(ion_handle* h, float fx, float fy, float fz);
+
+/* file::write / file::read, src/file.rs:12-30 (FILE_LAYOUT.txt).
+ * ion_load creates a handle from the file (not initialised, as the
+ * reference's file::read); cfg_inout supplies the fields the file does not
+ * store (graphics_active, units.k, units.prop, ...) and receives the result. */
+int ion_save(ion_handle* h, const char* path);
+int ion_load(const char* path, ion_config* cfg_inout, ion_handle** out);
+
+/* Float64 conservation sums over fluid cells: sum(rho), sum(Q). */
+int ion_conservation(ion_handle* h, double* mass, double* charge);
+
+int ion_set_field_mode(ion_handle* h, int mode);
+int ion_set_timing(ion_handle* h, int enable);
+int ion_get_timing(ion_handle* h, ion_timing* out);
+
+/* Device storage codec on host arrays (diagnostics): dir 0 encodes
+ * float -> u16 code, dir 1 decodes u16 -> float, for ION_FP16S/ION_FP16C. */
+int ion_device_codec(int float_type, int dir, const void* src, void* dst, uint64_t n);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* ion_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

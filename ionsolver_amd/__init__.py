@@ -1,0 +1,11 @@
+"""MI355X (gfx950) backend of IonSolver's extended-LBM MHD time step.
+
+The compute path is libionsolver_hip.so (hand-written HIP kernels behind the C
+ABI of include/ionsolver.h); this package is the host-side mirror of the
+reference's Lbm/LbmConfig API over that ABI, plus scenario setups.
+"""
+from .lbm import (Field, FieldMode, FloatType, GraphicsConfig, IonError, Lbm, LbmConfig, Propellant,
+                  RelaxationTime, Units, VelocitySet, device_codec, load_library, LIB_PATH, ABI)
+
+__all__ = ["Field", "FieldMode", "FloatType", "GraphicsConfig", "IonError", "Lbm", "LbmConfig", "Propellant",
+           "RelaxationTime", "Units", "VelocitySet", "device_codec", "load_library", "LIB_PATH", "ABI"]

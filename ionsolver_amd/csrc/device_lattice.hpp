@@ -1,0 +1,200 @@
+// Device-side lattice arithmetic for the gfx950 kernels: velocity-set tables,
+// DDF storage codecs, equilibria, moments and Guo forcing. Every function is
+// evaluated in the operation order of the reference kernels
+// (src/kernels/sim_kernels.cl) so that results agree exactly with a strict
+// IEEE restatement; the sources must be compiled with -ffp-contract=off.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ion {
+
+// ---------------------------------------------------------------------------
+// Velocity sets, reference direction ordering (sim_kernels.cl:336-339,
+// neighbor order :277-286, weights src/lbm/domain.rs:757-768).
+// ---------------------------------------------------------------------------
+template <int Q> struct VS;
+template <> struct VS<9> {
+    static constexpr int dims = 2;
+    static constexpr int cx[9] = {0, 1, -1, 0, 0, 1, -1, 1, -1};
+    static constexpr int cy[9] = {0, 0, 0, 1, -1, 1, -1, -1, 1};
+    static constexpr int cz[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    static constexpr int cls[9] = {0, 1, 1, 1, 1, 2, 2, 2, 2};
+    static constexpr float w0 = 1.0f / 2.25f, ws = 1.0f / 9.0f, we = 1.0f / 36.0f;
+};
+template <> struct VS<15> {
+    static constexpr int dims = 3;
+    static constexpr int cx[15] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
+    static constexpr int cy[15] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, -1, 1, 1, -1};
+    static constexpr int cz[15] = {0, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, 1, -1, 1, -1};
+    static constexpr int cls[15] = {0, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2};
+    static constexpr float w0 = 1.0f / 4.5f, ws = 1.0f / 9.0f, we = 1.0f / 72.0f;
+};
+template <> struct VS<19> {
+    static constexpr int dims = 3;
+    static constexpr int cx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0};
+    static constexpr int cy[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1};
+    static constexpr int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1};
+    static constexpr int cls[19] = {0, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
+    static constexpr float w0 = 1.0f / 3.0f, ws = 1.0f / 18.0f, we = 1.0f / 36.0f;
+};
+
+template <int Q> __device__ __forceinline__ constexpr float weight(int i) {
+    return VS<Q>::cls[i] == 0 ? VS<Q>::w0 : (VS<Q>::cls[i] == 1 ? VS<Q>::ws : VS<Q>::we);
+}
+
+// Signed sum c_i . (X, Y, Z) over the non-zero components, left to right,
+// the first term carrying its own sign (e.g. "-ux+uy+uz"). Multiplications by
+// the unit direction components are exact and are not performed.
+template <int Q, int I> __device__ __forceinline__ float proj(float X, float Y, float Z) {
+    constexpr int a = VS<Q>::cx[I], b = VS<Q>::cy[I], c = VS<Q>::cz[I];
+    float s = 0.0f;
+    bool first = true;
+    if (a != 0) { s = a > 0 ? X : -X; first = false; }
+    if (b != 0) { s = first ? (b > 0 ? Y : -Y) : (b > 0 ? s + Y : s - Y); first = false; }
+    if (c != 0) { s = first ? (c > 0 ? Z : -Z) : (c > 0 ? s + Z : s - Z); }
+    return s;
+}
+
+template <int Q, int I> struct Unroll {
+    template <class F> __device__ __forceinline__ static void run(F&& f) {
+        Unroll<Q, I - 1>::run(f);
+        f(std::integral_constant<int, I - 1>{});
+    }
+};
+template <int Q> struct Unroll<Q, 0> {
+    template <class F> __device__ __forceinline__ static void run(F&&) {}
+};
+// static_for over i = 0..N-1 with a compile-time index
+template <int N, class F> __device__ __forceinline__ void static_for(F&& f) { Unroll<N, N>::run(f); }
+
+// calculate_f_eq, sim_kernels.cl:155-207
+template <int Q> __device__ __forceinline__ void f_eq(float rho, float ux, float uy, float uz, float* feq) {
+    const float c3 = -3.0f * ((ux * ux + uy * uy) + uz * uz);
+    const float rhom1 = rho - 1.0f;
+    ux *= 3.0f;
+    uy *= 3.0f;
+    uz *= 3.0f;
+    feq[0] = VS<Q>::w0 * __builtin_fmaf(rho, 0.5f * c3, rhom1);
+    const float rhos = VS<Q>::ws * rho, rhoe = VS<Q>::we * rho;
+    const float rhom1s = VS<Q>::ws * rhom1, rhom1e = VS<Q>::we * rhom1;
+    static_for<(Q - 1) / 2>([&](auto p) {
+        constexpr int i = 2 * decltype(p)::value + 1;
+        const float s = proj<Q, i>(ux, uy, uz);
+        const float rw = VS<Q>::cls[i] == 1 ? rhos : rhoe;
+        const float rm = VS<Q>::cls[i] == 1 ? rhom1s : rhom1e;
+        const float sq = __builtin_fmaf(s, s, c3);
+        feq[i] = __builtin_fmaf(rw, __builtin_fmaf(0.5f, sq, s), rm);
+        feq[i + 1] = __builtin_fmaf(rw, __builtin_fmaf(0.5f, sq, -s), rm);
+    });
+}
+
+// calculate_rho_u, sim_kernels.cl:208-233: density summed in index order then
+// +1; each velocity component alternates + and - over the direction pairs
+// with a non-zero component, in index order.
+template <int Q> __device__ __forceinline__ void rho_u(const float* f, float& rho, float& ux, float& uy, float& uz) {
+    float r = f[0];
+    static_for<Q - 1>([&](auto k) { r += f[decltype(k)::value + 1]; });
+    r += 1.0f;
+    float u[3] = {0.0f, 0.0f, 0.0f};
+    static_for<3>([&](auto kk) {
+        constexpr int k = decltype(kk)::value;
+        float acc = 0.0f;
+        bool first = true;
+        static_for<(Q - 1) / 2>([&](auto p) {
+            constexpr int i = 2 * decltype(p)::value + 1;
+            constexpr int c = k == 0 ? VS<Q>::cx[i] : (k == 1 ? VS<Q>::cy[i] : VS<Q>::cz[i]);
+            if (c != 0) {
+                const float a = c > 0 ? f[i] : f[i + 1];
+                const float b = c > 0 ? f[i + 1] : f[i];
+                acc = first ? a - b : (acc + a) - b;
+                first = false;
+            }
+        });
+        u[k] = acc;
+    });
+    rho = r;
+    ux = u[0] / r;
+    uy = u[1] / r;
+    uz = u[2] / r;
+}
+
+// calculate_forcing_terms (Guo), sim_kernels.cl:366-378. The unit direction
+// components enter through exact sign selections.
+template <int Q> __device__ __forceinline__ void forcing(float ux, float uy, float uz, float fx, float fy, float fz, float* Fin) {
+    float uF;
+    if (VS<Q>::dims == 2) uF = -0.33333334f * __builtin_fmaf(ux, fx, uy * fy);
+    else uF = -0.33333334f * __builtin_fmaf(ux, fx, __builtin_fmaf(uy, fy, uz * fz));
+    Fin[0] = (9.0f * VS<Q>::w0) * uF;
+    static_for<Q - 1>([&](auto k) {
+        constexpr int i = decltype(k)::value + 1;
+        constexpr float wi = weight<Q>(i);
+        Fin[i] = (9.0f * wi) * __builtin_fmaf(proj<Q, i>(fx, fy, fz), proj<Q, i>(ux, uy, uz) + 0.33333334f, uF);
+    });
+}
+
+// calculate_a_eq (D3Q7), sim_kernels.cl:390-396
+__device__ __forceinline__ void a_eq(float Qv, float ux, float uy, float uz, float* qeq) {
+    const float wsT4 = 0.5f * Qv, wsTm1 = 0.125f * (Qv - 1.0f);
+    qeq[0] = __builtin_fmaf(0.25f, Qv, -0.25f);
+    qeq[1] = __builtin_fmaf(wsT4, ux, wsTm1);
+    qeq[2] = __builtin_fmaf(wsT4, -ux, wsTm1);
+    qeq[3] = __builtin_fmaf(wsT4, uy, wsTm1);
+    qeq[4] = __builtin_fmaf(wsT4, -uy, wsTm1);
+    qeq[5] = __builtin_fmaf(wsT4, uz, wsTm1);
+    qeq[6] = __builtin_fmaf(wsT4, -uz, wsTm1);
+}
+
+// OpenCL clamp(x, lo, hi) = fmin(fmax(x, lo), hi)
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+
+// ---------------------------------------------------------------------------
+// DDF storage formats (load/store macros, src/lbm/domain.rs:773-784)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t fp16c_encode(float x) {
+    // custom 1-4-11 format, bias 15 (sim_kernels.cl:79-84)
+    // a NaN of either sign is encoded as the positive quiet NaN (0/0 on the GPU)
+    const uint32_t b = (x != x ? 0x7FC00000u : __float_as_uint(x)) + 0x00000800u;
+    const uint32_t e = (b & 0x7F800000u) >> 23;
+    const uint32_t m = b & 0x007FFFFFu;
+    const uint32_t sgn = (b & 0x80000000u) >> 16;
+    const uint32_t nrm = (((e - 112u) << 11) & 0x7800u) | (m >> 12);
+    const uint32_t den = (((0x007FF800u + m) >> ((124u - e) & 31u)) + 1u) >> 1;
+    // branch-free selection of the normal / denormal / flushed case
+    const uint32_t mn = 0u - (uint32_t)(e > 112u);
+    const uint32_t md = 0u - (uint32_t)((e < 113u) & (e > 100u));
+    return (uint16_t)(sgn | (nrm & mn) | (den & md));
+}
+__device__ __forceinline__ float fp16c_decode(uint16_t x) {
+    // sim_kernels.cl:85-90; denormal codes are m * 2^-25 (exact)
+    const uint32_t e = (x & 0x7800u) >> 11;
+    const uint32_t m = x & 0x07FFu;
+    const float v = e != 0u ? __uint_as_float(((e + 112u) << 23) | (m << 12)) : (float)m * 0x1p-25f;
+    return (x & 0x8000u) ? -v : v;
+}
+
+struct F32 {
+    typedef float T;
+    static __device__ __forceinline__ float ld(float v) { return v; }
+    static __device__ __forceinline__ float st(float x) { return x; }
+};
+struct F16S {  // IEEE binary16 scaled by 2^15: vload_half*3.0517578E-5f / vstore_half_rte(x*32768.0f)
+    typedef uint16_t T;
+    static __device__ __forceinline__ float ld(uint16_t v) {
+        return (float)__builtin_bit_cast(_Float16, v) * 3.0517578E-5f;
+    }
+    static __device__ __forceinline__ uint16_t st(float x) {
+        const float y = x * 32768.0f;
+        // the sign bit is re-applied because the compiler may fuse the scaling
+        // into v_fma_mixlo_f16 (y*1 + 0), which turns -0 into +0
+        const uint16_t h = __builtin_bit_cast(uint16_t, (_Float16)y) | (uint16_t)((__float_as_uint(y) >> 16) & 0x8000u);
+        return y != y ? (uint16_t)0x7E00u : h;  // canonical NaN
+    }
+};
+struct F16C {
+    typedef uint16_t T;
+    static __device__ __forceinline__ float ld(uint16_t v) { return fp16c_decode(v); }
+    static __device__ __forceinline__ uint16_t st(float x) { return fp16c_encode(x); }
+};
+
+}  // namespace ion

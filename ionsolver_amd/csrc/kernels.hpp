@@ -1,0 +1,80 @@
+// Kernel argument blocks and launchers shared by the host runtime
+// (lbm_host.cpp) and the gfx950 kernels (kernels_lbm.hip, kernels_field.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ion {
+
+enum StoreKind { STORE_FP16S = 0, STORE_FP16C = 1, STORE_FP32 = 2 };
+
+// Arguments of initialize / stream_collide / update_fields. Buffers are the
+// reference's device buffers (src/lbm/domain.rs:151-211), all SoA:
+// scalar fields [N], vector fields [3][N], DDFs [q][N].
+struct LbmArgs {
+    void* fi;
+    float* rho;
+    float* u;
+    const uint8_t* flags;
+    const float* F;       // force field (FORCE_FIELD), else null
+    float* e_stat;
+    float* b_stat;
+    float* e_dyn;
+    float* b_dyn;
+    void* fqi;            // D3Q7 charge DDFs
+    void* ei;             // electron-gas DDFs
+    float* q;             // cell charge
+    float4* lod_seg;      // level-1 LOD row-segment sums [N / lod_bx]
+    uint32_t nx, ny, nz;
+    uint32_t pad0;
+    uint64_t N;
+    uint32_t odd;         // t % 2 (esoteric-pull layout)
+    uint32_t eqb, vforce, ffield, upd;  // extension switches
+    uint32_t lod_bx;      // LOD block side along x (cells)
+    float fx, fy, fz;     // volume force
+    float w;              // DEF_W
+    float c;              // DEF_C
+    float kkge, wq;       // DEF_KKGE, DEF_WQ
+    float ils;            // 1 / cells per LOD block
+};
+
+// Arguments of the E/B field update.
+struct FieldArgs {
+    const float* e_stat;
+    const float* b_stat;
+    float* e_dyn;
+    float* b_dyn;
+    const uint8_t* flags;
+    const float* q;
+    const float* u;
+    const float4* src_pq;   // LOD sources: centre x,y,z and charge
+    const float4* src_vd;   // LOD sources: mean velocity and LOD index (bits)
+    const uint32_t* src_count;
+    uint32_t nx, ny, nz;
+    uint32_t D;
+    uint64_t N;
+    uint32_t dsx, dsy, dsz; // near-field block side
+    uint32_t near;          // near-field block has more than one cell
+    float ke, kmu;
+};
+
+// kernels_lbm.hip
+hipError_t launch_initialize(int q, int store, bool mhd, const LbmArgs& a, hipStream_t s);
+hipError_t launch_stream_collide(int q, int store, bool mhd, bool trt, bool lod, uint32_t block,
+                                 const LbmArgs& a, hipStream_t s);
+hipError_t launch_update_fields(int q, int store, const LbmArgs& a, hipStream_t s);
+// Level-2 LOD tree: one block per top-level LOD.
+hipError_t launch_lod_reduce(const float4* seg, float4* qu_lod, uint32_t nx, uint32_t ny, uint32_t nz,
+                             uint32_t D, hipStream_t s);
+hipError_t launch_fill_f32(float* p, uint64_t n, float v, hipStream_t s);
+hipError_t launch_codec(int store, int dir, const void* in, void* out, uint64_t n, hipStream_t s);
+
+// kernels_field.hip
+hipError_t launch_prep_sources(const float4* qu_lod, uint32_t lo, uint32_t hi, uint32_t nx, uint32_t ny,
+                               uint32_t nz, uint32_t D, float4* src_pq, float4* src_vd, uint32_t* count,
+                               hipStream_t s);
+hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s);
+hipError_t launch_sum_f64(const float* x, const uint8_t* flags, uint64_t n, double* partial, uint32_t nblocks,
+                          hipStream_t s);
+
+}  // namespace ion

@@ -1,0 +1,227 @@
+// gfx950 kernels of the dynamic E/B field update (update_e_b_dynamic,
+// sim_kernels.cl:897-993) and of the float64 conservation sums.
+//
+// The field at every fluid cell is a sum over the level-of-detail (LOD)
+// sources of the own domain: E += q r/|r|^3, B += q v x r/|r|^3. The
+// sources are compacted into a list (centre, charge, mean velocity, LOD
+// index) once per step, then staged through LDS in tiles; every work-item
+// accumulates the fields of its cells in registers, sweeping the sources
+// in ascending LOD index as the reference loop does.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace ion {
+
+// ---------------------------------------------------------------------------
+// Source list: LOD indices d in [lo, hi) with a non-zero charge, in index
+// order. A source of zero charge adds exactly zero to every cell, so
+// dropping it leaves all fields unchanged.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_prep_sources(const float4* __restrict__ qu, uint32_t lo, uint32_t hi,
+                                                       uint32_t nd, float dsx, float dsy, float dsz,
+                                                       float4* __restrict__ src_pq, float4* __restrict__ src_vd,
+                                                       uint32_t* __restrict__ count) {
+    __shared__ uint32_t wave_tot[16];
+    __shared__ uint32_t base_sh;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
+    if (tid == 0) base_sh = 0;
+    __syncthreads();
+    for (uint32_t c0 = lo; c0 < hi; c0 += 1024) {
+        const uint32_t d = c0 + tid;
+        float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        bool keep = false;
+        if (d < hi) {
+            e = qu[d];
+            keep = e.x != 0.0f;
+        }
+        const uint64_t mask = __ballot(keep);
+        const uint32_t pre = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) wave_tot[wv] = __popcll(mask);
+        __syncthreads();
+        uint32_t off = base_sh;
+        for (uint32_t k = 0; k < wv; k++) off += wave_tot[k];
+        if (keep) {
+            const uint32_t o = off + pre;
+            const uint32_t t = d % (nd * nd);
+            // lod_coordinates, sim_kernels.cl:442-447
+            const float lx = (float)(t % nd) * dsx + 0.5f * dsx;
+            const float ly = (float)(t / nd) * dsy + 0.5f * dsy;
+            const float lz = (float)(d / (nd * nd)) * dsz + 0.5f * dsz;
+            src_pq[o] = make_float4(lx, ly, lz, e.x);
+            src_vd[o] = make_float4(e.y, e.z, e.w, __uint_as_float(d));
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t tot = 0;
+            for (uint32_t k = 0; k < 16; k++) tot += wave_tot[k];
+            base_sh += tot;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *count = base_sh;
+}
+
+constexpr uint32_t FIELD_TILE = 512;
+constexpr uint32_t FIELD_BLOCK = 256;
+
+// One interaction in the reference's arithmetic: pre = r / (sqrt(r.r))^3
+// with correctly rounded sqrt and division, e += q*pre, b += q*(v x pre).
+__device__ __forceinline__ void interact_exact(float q, float vx, float vy, float vz, float rx, float ry, float rz,
+                                               float& ex, float& ey, float& ez, float& bx, float& by, float& bz) {
+    const float s = sqrtf((rx * rx + ry * ry) + rz * rz);
+    const float cbm = (s * s) * s;
+    const float prx = rx / cbm, pry = ry / cbm, prz = rz / cbm;
+    ex = ex + q * prx;
+    ey = ey + q * pry;
+    ez = ez + q * prz;
+    const float crx = vy * prz - vz * pry, cry = vz * prx - vx * prz, crz = vx * pry - vy * prx;
+    bx = bx + q * crx;
+    by = by + q * cry;
+    bz = bz + q * crz;
+}
+
+template <bool FAST, bool NEAR, int CPT>
+__global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b(FieldArgs a) {
+    __shared__ float4 s_pq[FIELD_TILE];
+    __shared__ float4 s_vd[FIELD_TILE];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t NX = a.nx, NY = a.ny, NZ = a.nz;
+    const uint32_t nd = 1u << a.D;
+    const uint32_t lbx = NX / nd, lby = NY / nd, lbz = NZ / nd;
+
+    float px[CPT], py[CPT], pz[CPT];
+    float ex[CPT], ey[CPT], ez[CPT], bx[CPT], by[CPT], bz[CPT];
+    uint32_t own[CPT], cell[CPT];
+    bool act[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        const uint64_t n64 = (uint64_t)blockIdx.x * (FIELD_BLOCK * CPT) + (uint64_t)k * FIELD_BLOCK + tid;
+        act[k] = n64 < a.N;
+        const uint32_t n = act[k] ? (uint32_t)n64 : 0u;
+        cell[k] = n;
+        if (act[k]) act[k] = (a.flags[n] & 0x1F) != 0x01;
+        const uint32_t z = n / (NX * NY), t = n - z * NX * NY, y = t / NX, x = t - y * NX;
+        px[k] = (float)x;
+        py[k] = (float)y;
+        pz[k] = (float)z;
+        own[k] = x / lbx + (y / lby + z / lbz * nd) * nd;  // lod_index, sim_kernels.cl:425-433
+        ex[k] = ey[k] = ez[k] = bx[k] = by[k] = bz[k] = 0.0f;
+        if (NEAR && act[k]) {
+            // individual cells of the own near-field block, x, y, z nesting
+            // as in sim_kernels.cl:913-938
+            const uint32_t x0 = (x / a.dsx) * a.dsx, y0 = (y / a.dsy) * a.dsy, z0 = (z / a.dsz) * a.dsz;
+            const uint32_t xu = min(x0 + a.dsx, NX), yu = min(y0 + a.dsy, NY), zu = min(z0 + a.dsz, NZ);
+            for (uint32_t xx = x0; xx < xu; xx++)
+                for (uint32_t yy = y0; yy < yu; yy++)
+                    for (uint32_t zz = z0; zz < zu; zz++) {
+                        const uint32_t nc = xx + (yy + zz * NY) * NX;
+                        if (nc == n) continue;
+                        const float qc = a.q[nc];
+                        if (qc == 0.0f) continue;
+                        interact_exact(qc, a.u[nc], a.u[(uint64_t)nc + a.N], a.u[(uint64_t)nc + 2 * a.N],
+                                       px[k] - (float)xx, py[k] - (float)yy, pz[k] - (float)zz, ex[k], ey[k], ez[k],
+                                       bx[k], by[k], bz[k]);
+                    }
+        }
+    }
+
+    const uint32_t count = *a.src_count;
+    for (uint32_t base = 0; base < count; base += FIELD_TILE) {
+        const uint32_t m = min(FIELD_TILE, count - base);
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += FIELD_BLOCK) {
+            s_pq[i] = a.src_pq[base + i];
+            s_vd[i] = a.src_vd[base + i];
+        }
+        __syncthreads();
+        for (uint32_t s = 0; s < m; s++) {
+            const float4 pq = s_pq[s];
+            const float4 vd = s_vd[s];
+            const uint32_t d = __float_as_uint(vd.w);
+#pragma unroll
+            for (int k = 0; k < CPT; k++) {
+                const float rx = px[k] - pq.x, ry = py[k] - pq.y, rz = pz[k] - pq.z;
+                if (FAST) {
+                    const float r2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    float t = pq.w * (inv * inv * inv);
+                    t = (d == own[k]) ? 0.0f : t;
+                    ex[k] = __builtin_fmaf(t, rx, ex[k]);
+                    ey[k] = __builtin_fmaf(t, ry, ey[k]);
+                    ez[k] = __builtin_fmaf(t, rz, ez[k]);
+                    const float crx = __builtin_fmaf(vd.y, rz, -vd.z * ry);
+                    const float cry = __builtin_fmaf(vd.z, rx, -vd.x * rz);
+                    const float crz = __builtin_fmaf(vd.x, ry, -vd.y * rx);
+                    bx[k] = __builtin_fmaf(t, crx, bx[k]);
+                    by[k] = __builtin_fmaf(t, cry, by[k]);
+                    bz[k] = __builtin_fmaf(t, crz, bz[k]);
+                } else {
+                    if (d != own[k])
+                        interact_exact(pq.w, vd.x, vd.y, vd.z, rx, ry, rz, ex[k], ey[k], ez[k], bx[k], by[k], bz[k]);
+                }
+            }
+        }
+    }
+
+    const uint64_t N = a.N;
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        if (!act[k]) continue;
+        const uint32_t n = cell[k];
+        a.e_dyn[n] = a.e_stat[n] + a.ke * ex[k];
+        a.e_dyn[N + n] = a.e_stat[N + n] + a.ke * ey[k];
+        a.e_dyn[2 * N + n] = a.e_stat[2 * N + n] + a.ke * ez[k];
+        a.b_dyn[n] = a.b_stat[n] + a.kmu * bx[k];
+        a.b_dyn[N + n] = a.b_stat[N + n] + a.kmu * by[k];
+        a.b_dyn[2 * N + n] = a.b_stat[2 * N + n] + a.kmu * bz[k];
+    }
+}
+
+// Float64 sum of x over fluid cells (flags & 0x1F != 0x01): one partial per
+// block, summed on the host in block order.
+__global__ void __launch_bounds__(256) k_sum_f64(const float* __restrict__ x, const uint8_t* __restrict__ flags,
+                                                 uint64_t n, double* __restrict__ partial) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        if ((flags[i] & 0x1F) != 0x01) acc += (double)x[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t s = 128; s >= 1; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+hipError_t launch_prep_sources(const float4* qu_lod, uint32_t lo, uint32_t hi, uint32_t nx, uint32_t ny, uint32_t nz,
+                               uint32_t D, float4* src_pq, float4* src_vd, uint32_t* count, hipStream_t s) {
+    const uint32_t nd = 1u << D;
+    hipLaunchKernelGGL(k_prep_sources, dim3(1), dim3(1024), 0, s, qu_lod, lo, hi, nd, (float)(nx / nd),
+                       (float)(ny / nd), (float)(nz / nd), src_pq, src_vd, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s) {
+    constexpr int CPT_EXACT = 1, CPT_FAST = 2;
+    const int cpt = fast ? CPT_FAST : CPT_EXACT;
+    const uint64_t per = (uint64_t)FIELD_BLOCK * cpt;
+    const dim3 g((uint32_t)((a.N + per - 1) / per)), b(FIELD_BLOCK);
+    if (fast) {
+        if (a.near) hipLaunchKernelGGL((k_update_e_b<true, true, CPT_FAST>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_update_e_b<true, false, CPT_FAST>), g, b, 0, s, a);
+    } else {
+        if (a.near) hipLaunchKernelGGL((k_update_e_b<false, true, CPT_EXACT>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_update_e_b<false, false, CPT_EXACT>), g, b, 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_f64(const float* x, const uint8_t* flags, uint64_t n, double* partial, uint32_t nblocks,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_f64, dim3(nblocks), dim3(256), 0, s, x, flags, n, partial);
+    return hipGetLastError();
+}
+
+}  // namespace ion

@@ -1,0 +1,467 @@
+// gfx950 kernels of the lattice part of the time step: initialize,
+// stream_collide (fluid + electron gas + D3Q7 charge advection + Lorentz
+// force + level-of-detail charge sums) and update_fields.
+//
+// One work-item per cell, DDFs in the reference's SoA layout
+// (index_f(n, i) = i*N + n, sim_kernels.cl:152-154), streamed in place by
+// the esoteric-pull scheme (sim_kernels.cl:234-247), so every DDF slot is
+// read once and written once per step by exactly one work-item.
+#include <hip/hip_runtime.h>
+
+#include "device_lattice.hpp"
+#include "kernels.hpp"
+
+namespace ion {
+
+struct Nbr {
+    uint32_t x0, xp, xm, y0, yp, ym, z0, zp, zm;
+};
+
+// calculate_indices, sim_kernels.cl:248-259 (periodic wrap)
+__device__ __forceinline__ Nbr neighbours(uint32_t n, uint32_t NX, uint32_t NY, uint32_t NZ) {
+    const uint32_t nxy = NX * NY;
+    const uint32_t z = n / nxy;
+    const uint32_t t = n - z * nxy;
+    const uint32_t y = t / NX;
+    const uint32_t x = t - y * NX;
+    Nbr b;
+    b.x0 = x;
+    b.xp = (x + 1u == NX) ? 0u : x + 1u;
+    b.xm = (x == 0u) ? NX - 1u : x - 1u;
+    b.y0 = y * NX;
+    b.yp = ((y + 1u == NY) ? 0u : y + 1u) * NX;
+    b.ym = ((y == 0u) ? NY - 1u : y - 1u) * NX;
+    b.z0 = z * nxy;
+    b.zp = ((z + 1u == NZ) ? 0u : z + 1u) * nxy;
+    b.zm = ((z == 0u) ? NZ - 1u : z - 1u) * nxy;
+    return b;
+}
+
+template <int Q, int I> __device__ __forceinline__ uint32_t nb_index(const Nbr& b) {
+    constexpr int cx = VS<Q>::cx[I], cy = VS<Q>::cy[I], cz = VS<Q>::cz[I];
+    const uint32_t x = cx == 0 ? b.x0 : (cx > 0 ? b.xp : b.xm);
+    const uint32_t y = cy == 0 ? b.y0 : (cy > 0 ? b.yp : b.ym);
+    const uint32_t z = cz == 0 ? b.z0 : (cz > 0 ? b.zp : b.zm);
+    return x + y + z;
+}
+
+// Esoteric-pull load/store of an NQ-direction lattice (load_f/store_f and
+// load_a/store_a, sim_kernels.cl:234-247, 397-410). Direction pair (i, i+1)
+// of cell n lives in slots (n, i+1) and (j[i], i) on even steps and in
+// (n, i) and (j[i], i+1) on odd steps.
+template <int Q, int NQ, class S>
+__device__ __forceinline__ void load_ddfs(const typename S::T* __restrict__ buf, uint64_t N, uint32_t n,
+                                          const Nbr& b, uint32_t odd, float* fh) {
+    fh[0] = S::ld(buf[n]);
+    static_for<(NQ - 1) / 2>([&](auto p) {
+        constexpr int i = 2 * decltype(p)::value + 1;
+        const uint32_t j = nb_index<Q, i>(b);
+        const uint64_t so = odd ? (uint64_t)i : (uint64_t)(i + 1);
+        const uint64_t sn = odd ? (uint64_t)(i + 1) : (uint64_t)i;
+        fh[i] = S::ld(buf[so * N + n]);
+        fh[i + 1] = S::ld(buf[sn * N + j]);
+    });
+}
+template <int Q, int NQ, class S>
+__device__ __forceinline__ void store_ddfs(typename S::T* __restrict__ buf, uint64_t N, uint32_t n, const Nbr& b,
+                                           uint32_t odd, const float* fh) {
+    buf[n] = S::st(fh[0]);
+    static_for<(NQ - 1) / 2>([&](auto p) {
+        constexpr int i = 2 * decltype(p)::value + 1;
+        const uint32_t j = nb_index<Q, i>(b);
+        const uint64_t so = odd ? (uint64_t)i : (uint64_t)(i + 1);
+        const uint64_t sn = odd ? (uint64_t)(i + 1) : (uint64_t)i;
+        buf[sn * N + j] = S::st(fh[i]);
+        buf[so * N + n] = S::st(fh[i + 1]);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// initialize, sim_kernels.cl:760-832 (writes the t = 1 layout)
+// ---------------------------------------------------------------------------
+template <int Q, class S, bool MHD>
+__global__ void __launch_bounds__(256) k_initialize(LbmArgs a) {
+    const uint64_t n64 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n64 >= a.N) return;
+    const uint32_t n = (uint32_t)n64;
+    const uint64_t N = a.N;
+    const Nbr b = neighbours(n, a.nx, a.ny, a.nz);
+    const uint8_t bo = a.flags[n] & 0x1F;
+    if (bo == 0x01) {
+        a.u[n] = 0.0f;
+        a.u[N + n] = 0.0f;
+        a.u[2 * N + n] = 0.0f;
+        if (MHD) a.q[n] = 0.0f;
+    }
+    const float ux = a.u[n], uy = a.u[N + n], uz = a.u[2 * N + n];
+    float fe[Q];
+    f_eq<Q>(a.rho[n], ux, uy, uz, fe);
+    store_ddfs<Q, Q, S>((typename S::T*)a.fi, N, n, b, 1u, fe);
+    if (MHD) {
+        float qeq[7];
+        a_eq(a.q[n], ux, uy, uz, qeq);
+        store_ddfs<19, 7, S>((typename S::T*)a.fqi, N, n, b, 1u, qeq);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            a.b_dyn[k * N + n] = a.b_stat[k * N + n];
+            a.e_dyn[k * N + n] = a.e_stat[k * N + n];
+        }
+        f_eq<Q>(0.0f, ux, uy, uz, fe);
+        store_ddfs<Q, Q, S>((typename S::T*)a.ei, N, n, b, 1u, fe);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stream_collide, sim_kernels.cl:465-758
+// LOD: the block covers a contiguous range of whole LOD row segments
+// (lod_bx consecutive cells of one row); the per-cell contributions are
+// summed per segment by the pairwise tree in LDS and written to lod_seg.
+// ---------------------------------------------------------------------------
+template <int Q, class S, bool MHD, bool TRT, bool LOD>
+__global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
+    extern __shared__ float4 lds_lod[];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t n64 = (uint64_t)blockIdx.x * blockDim.x + tid;
+    const uint64_t N = a.N;
+    const uint32_t n = (uint32_t)n64;
+    bool active = n64 < N;
+    uint8_t bo = 0x01;
+    if (active) bo = a.flags[n] & 0x1F;
+    active = active && bo != 0x01;
+    float4 lod = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    typedef typename S::T T;
+
+    if (active) {
+        const bool is_e = a.eqb && bo == 0x02;
+        const Nbr b = neighbours(n, a.nx, a.ny, a.nz);
+        float fhn[Q];
+        load_ddfs<Q, Q, S>((const T*)a.fi, N, n, b, a.odd, fhn);
+        float rhon, uxn, uyn, uzn;
+        if (is_e) {
+            rhon = a.rho[n];
+            uxn = a.u[n];
+            uyn = a.u[N + n];
+            uzn = a.u[2 * N + n];
+        } else {
+            rho_u<Q>(fhn, rhon, uxn, uyn, uzn);
+        }
+        float fxn = a.fx, fyn = a.fy, fzn = a.fz;
+        float Fin[Q], feq[Q];
+        const float w = a.w;
+        const float c_tau = __builtin_fmaf(w, -0.5f, 1.0f);
+        if (a.ffield) {
+            fxn += a.F[n];
+            fyn += a.F[N + n];
+            fzn += a.F[2 * N + n];
+        }
+        if (MHD) {
+            const float Bx = a.b_dyn[n], By = a.b_dyn[N + n], Bz = a.b_dyn[2 * N + n];
+            const float Ex = a.e_dyn[n], Ey = a.e_dyn[N + n], Ez = a.e_dyn[2 * N + n];
+            // electron gas part 1
+            float ehn[Q];
+            load_ddfs<Q, Q, S>((const T*)a.ei, N, n, b, a.odd, ehn);
+            float rhon_e, uxn_e, uyn_e, uzn_e;
+            rho_u<Q>(ehn, rhon_e, uxn_e, uyn_e, uzn_e);
+            // charge advection part 1 (D3Q7 subset of D3Q19 neighbours)
+            float qhn[7];
+            load_ddfs<19, 7, S>((const T*)a.fqi, N, n, b, a.odd, qhn);
+            float rhon_q = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 7; i++) rhon_q += qhn[i];
+            rhon_q += 1.0f;
+            // charge advection part 2
+            a.q[n] = rhon_q - rhon_e;
+            float qeq[7];
+            a_eq(rhon_q, uxn, uyn, uzn, qeq);
+            const float wq = a.wq;
+#pragma unroll
+            for (int i = 0; i < 7; i++) qhn[i] = __builtin_fmaf(1.0f - wq, qhn[i], wq * qeq[i]);
+            store_ddfs<19, 7, S>((T*)a.fqi, N, n, b, a.odd, qhn);
+            // electron gas part 2: F = -rho_e (E + u_e x B), Guo forcing, SRT
+            const float crx = uyn_e * Bz - uzn_e * By;
+            const float cry = uzn_e * Bx - uxn_e * Bz;
+            const float crz = uxn_e * By - uyn_e * Bx;
+            const float efx = -rhon_e * (Ex + crx), efy = -rhon_e * (Ey + cry), efz = -rhon_e * (Ez + crz);
+            const float rho2_e = 0.5f / (rhon_e * a.kkge);
+            uxn_e = clampf(__builtin_fmaf(efx, rho2_e, uxn_e), -a.c, a.c);
+            uyn_e = clampf(__builtin_fmaf(efy, rho2_e, uyn_e), -a.c, a.c);
+            uzn_e = clampf(__builtin_fmaf(efz, rho2_e, uzn_e), -a.c, a.c);
+            forcing<Q>(uxn_e, uyn_e, uzn_e, efx, efy, efz, Fin);
+            f_eq<Q>(rhon_e, uxn_e, uyn_e, uzn_e, feq);
+#pragma unroll
+            for (int i = 0; i < Q; i++) {
+                const float fi = Fin[i] * c_tau;
+                ehn[i] = is_e ? feq[i] : __builtin_fmaf(1.0f - w, ehn[i], __builtin_fmaf(w, feq[i], fi));
+            }
+            store_ddfs<Q, Q, S>((T*)a.ei, N, n, b, a.odd, ehn);
+            // EM force on the gas (plain multiply-add, as written)
+            fxn += rhon_q * (Ex + uyn * Bz - uzn * By);
+            fyn += rhon_q * (Ey + uzn * Bx - uxn * Bz);
+            fzn += rhon_q * (Ez + uxn * By - uyn * Bx);
+            if (LOD) lod = make_float4(rhon_q - rhon_e, uxn * a.ils, uyn * a.ils, uzn * a.ils);
+        }
+        if (a.vforce) {
+            const float rho2 = 0.5f / rhon;
+            uxn = clampf(__builtin_fmaf(fxn, rho2, uxn), -a.c, a.c);
+            uyn = clampf(__builtin_fmaf(fyn, rho2, uyn), -a.c, a.c);
+            uzn = clampf(__builtin_fmaf(fzn, rho2, uzn), -a.c, a.c);
+            forcing<Q>(uxn, uyn, uzn, fxn, fyn, fzn, Fin);
+        } else {
+            uxn = clampf(uxn, -a.c, a.c);
+            uyn = clampf(uyn, -a.c, a.c);
+            uzn = clampf(uzn, -a.c, a.c);
+#pragma unroll
+            for (int i = 0; i < Q; i++) Fin[i] = 0.0f;
+        }
+        if (a.upd && !is_e) {
+            a.rho[n] = rhon;
+            a.u[n] = uxn;
+            a.u[N + n] = uyn;
+            a.u[2 * N + n] = uzn;
+        }
+        f_eq<Q>(rhon, uxn, uyn, uzn, feq);
+        if (!TRT) {
+            const bool vf = a.vforce != 0;
+#pragma unroll
+            for (int i = 0; i < Q; i++) {
+                const float fi = vf ? Fin[i] * c_tau : Fin[i];
+                fhn[i] = is_e ? feq[i] : __builtin_fmaf(1.0f - w, fhn[i], __builtin_fmaf(w, feq[i], fi));
+            }
+        } else {
+            const float wp = w;
+            const float wm = 1.0f / (0.1875f / (1.0f / w - 0.5f) + 0.5f);
+            if (a.vforce) {
+                const float c_taup = __builtin_fmaf(wp, -0.25f, 0.5f), c_taum = __builtin_fmaf(wm, -0.25f, 0.5f);
+                float Fib[Q];
+                Fib[0] = Fin[0];
+#pragma unroll
+                for (int i = 1; i < Q; i += 2) {
+                    Fib[i] = Fin[i + 1];
+                    Fib[i + 1] = Fin[i];
+                }
+#pragma unroll
+                for (int i = 0; i < Q; i++) Fin[i] = __builtin_fmaf(c_taup, Fin[i] + Fib[i], c_taum * (Fin[i] - Fib[i]));
+            }
+            float fhb[Q], feb[Q];
+            fhb[0] = fhn[0];
+            feb[0] = feq[0];
+#pragma unroll
+            for (int i = 1; i < Q; i += 2) {
+                fhb[i] = fhn[i + 1];
+                fhb[i + 1] = fhn[i];
+                feb[i] = feq[i + 1];
+                feb[i + 1] = feq[i];
+            }
+#pragma unroll
+            for (int i = 0; i < Q; i++)
+                fhn[i] = is_e ? feq[i]
+                              : __builtin_fmaf(0.5f * wp, feq[i] - fhn[i] + feb[i] - fhb[i],
+                                               __builtin_fmaf(0.5f * wm, feq[i] - feb[i] - fhn[i] + fhb[i], fhn[i] + Fin[i]));
+        }
+        store_ddfs<Q, Q, S>((T*)a.fi, N, n, b, a.odd, fhn);
+    }
+
+    if (LOD) {
+        // level-1 pairwise tree over each LOD row segment
+        const uint32_t bx = a.lod_bx;
+        lds_lod[tid] = lod;
+        __syncthreads();
+        const uint32_t i = tid % bx;
+        uint32_t P = 1;
+        while (P < bx) P <<= 1;
+        for (uint32_t s = P >> 1; s >= 1; s >>= 1) {
+            if (i < s) {
+                const float4 v = lds_lod[tid];
+                const float4 o = (i + s < bx) ? lds_lod[tid + s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                lds_lod[tid] = make_float4(v.x + o.x, v.y + o.y, v.z + o.z, v.w + o.w);
+            }
+            __syncthreads();
+        }
+        if (i == 0 && n64 < N) a.lod_seg[n64 / bx] = lds_lod[tid];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// update_fields, sim_kernels.cl:834-859
+// ---------------------------------------------------------------------------
+template <int Q, class S>
+__global__ void __launch_bounds__(256) k_update_fields(LbmArgs a) {
+    const uint64_t n64 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n64 >= a.N) return;
+    const uint32_t n = (uint32_t)n64;
+    const uint64_t N = a.N;
+    if ((a.flags[n] & 0x1F) == 0x01) return;
+    const Nbr b = neighbours(n, a.nx, a.ny, a.nz);
+    float fhn[Q];
+    load_ddfs<Q, Q, S>((const typename S::T*)a.fi, N, n, b, a.odd, fhn);
+    float rhon, uxn, uyn, uzn;
+    rho_u<Q>(fhn, rhon, uxn, uyn, uzn);
+    a.rho[n] = rhon;
+    a.u[n] = clampf(uxn, -a.c, a.c);
+    a.u[N + n] = clampf(uyn, -a.c, a.c);
+    a.u[2 * N + n] = clampf(uzn, -a.c, a.c);
+}
+
+// ---------------------------------------------------------------------------
+// Level-2 LOD tree: block L sums the by*bz row-segment sums of LOD L
+// (index j = yy + zz*by) by the pairwise tree, and writes QU_lod[L]
+// (top-level LODs at offset 0 for a single domain, sim_kernels.cl:667-671).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_lod_reduce(const float4* __restrict__ seg, float4* __restrict__ qu,
+                                                     uint32_t NY, uint32_t nd, uint32_t by, uint32_t bz,
+                                                     uint32_t P) {
+    extern __shared__ float4 a4[];
+    const uint32_t L = blockIdx.x;
+    const uint32_t X = L % nd, Y = (L / nd) % nd, Z = L / (nd * nd);
+    const uint32_t cnt = by * bz;
+    for (uint32_t j = threadIdx.x; j < P; j += blockDim.x) {
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (j < cnt) {
+            const uint32_t yy = j % by, zz = j / by;
+            const uint64_t row = (uint64_t)(Z * bz + zz) * NY + (Y * by + yy);
+            v = seg[row * nd + X];
+        }
+        a4[j] = v;
+    }
+    __syncthreads();
+    for (uint32_t s = P >> 1; s >= 1; s >>= 1) {
+        for (uint32_t i = threadIdx.x; i < s; i += blockDim.x) {
+            const float4 v = a4[i], o = a4[i + s];
+            a4[i] = make_float4(v.x + o.x, v.y + o.y, v.z + o.z, v.w + o.w);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) qu[L] = a4[0];
+}
+
+__global__ void k_fill_f32(float* p, uint64_t n, float v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Device codecs on arrays (diagnostics: lets tests compare the device
+// conversion of every value class with the oracle's).
+template <class S>
+__global__ void k_codec(int dir, const void* in, void* out, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (dir == 0) ((uint16_t*)out)[i] = S::st(((const float*)in)[i]);
+    else ((float*)out)[i] = S::ld(((const uint16_t*)in)[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+template <int Q, class S>
+static hipError_t init_impl(bool mhd, const LbmArgs& a, hipStream_t s) {
+    const uint32_t g = blocks_for(a.N, 256);
+    if (mhd) hipLaunchKernelGGL((k_initialize<Q, S, true>), dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_initialize<Q, S, false>), dim3(g), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int Q, class S>
+static hipError_t sc_impl(bool mhd, bool trt, bool lod, uint32_t block, const LbmArgs& a, hipStream_t s) {
+    const uint32_t g = blocks_for(a.N, block);
+    const size_t sh = lod ? (size_t)block * sizeof(float4) : 0;
+#define ION_SC(M, T, L) hipLaunchKernelGGL((k_stream_collide<Q, S, M, T, L>), dim3(g), dim3(block), sh, s, a)
+    if (mhd) {
+        if (lod) { if (trt) ION_SC(true, true, true); else ION_SC(true, false, true); }
+        else { if (trt) ION_SC(true, true, false); else ION_SC(true, false, false); }
+    } else {
+        if (trt) ION_SC(false, true, false); else ION_SC(false, false, false);
+    }
+#undef ION_SC
+    return hipGetLastError();
+}
+
+template <int Q, class S>
+static hipError_t uf_impl(const LbmArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL((k_update_fields<Q, S>), dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_initialize(int q, int store, bool mhd, const LbmArgs& a, hipStream_t s) {
+    if (q == 19) {
+        if (store == STORE_FP32) return init_impl<19, F32>(mhd, a, s);
+        if (store == STORE_FP16S) return init_impl<19, F16S>(mhd, a, s);
+        if (store == STORE_FP16C) return init_impl<19, F16C>(mhd, a, s);
+    } else if (q == 15) {
+        if (store == STORE_FP32) return init_impl<15, F32>(mhd, a, s);
+        if (store == STORE_FP16S) return init_impl<15, F16S>(mhd, a, s);
+        if (store == STORE_FP16C) return init_impl<15, F16C>(mhd, a, s);
+    } else if (q == 9 && !mhd) {
+        if (store == STORE_FP32) return init_impl<9, F32>(false, a, s);
+        if (store == STORE_FP16S) return init_impl<9, F16S>(false, a, s);
+        if (store == STORE_FP16C) return init_impl<9, F16C>(false, a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_stream_collide(int q, int store, bool mhd, bool trt, bool lod, uint32_t block, const LbmArgs& a,
+                                 hipStream_t s) {
+    if (q == 19) {
+        if (store == STORE_FP32) return sc_impl<19, F32>(mhd, trt, lod, block, a, s);
+        if (store == STORE_FP16S) return sc_impl<19, F16S>(mhd, trt, lod, block, a, s);
+        if (store == STORE_FP16C) return sc_impl<19, F16C>(mhd, trt, lod, block, a, s);
+    } else if (q == 15) {
+        if (store == STORE_FP32) return sc_impl<15, F32>(mhd, trt, lod, block, a, s);
+        if (store == STORE_FP16S) return sc_impl<15, F16S>(mhd, trt, lod, block, a, s);
+        if (store == STORE_FP16C) return sc_impl<15, F16C>(mhd, trt, lod, block, a, s);
+    } else if (q == 9 && !mhd) {
+        if (store == STORE_FP32) return sc_impl<9, F32>(false, trt, false, block, a, s);
+        if (store == STORE_FP16S) return sc_impl<9, F16S>(false, trt, false, block, a, s);
+        if (store == STORE_FP16C) return sc_impl<9, F16C>(false, trt, false, block, a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_update_fields(int q, int store, const LbmArgs& a, hipStream_t s) {
+    if (q == 19) {
+        if (store == STORE_FP32) return uf_impl<19, F32>(a, s);
+        if (store == STORE_FP16S) return uf_impl<19, F16S>(a, s);
+        if (store == STORE_FP16C) return uf_impl<19, F16C>(a, s);
+    } else if (q == 15) {
+        if (store == STORE_FP32) return uf_impl<15, F32>(a, s);
+        if (store == STORE_FP16S) return uf_impl<15, F16S>(a, s);
+        if (store == STORE_FP16C) return uf_impl<15, F16C>(a, s);
+    } else if (q == 9) {
+        if (store == STORE_FP32) return uf_impl<9, F32>(a, s);
+        if (store == STORE_FP16S) return uf_impl<9, F16S>(a, s);
+        if (store == STORE_FP16C) return uf_impl<9, F16C>(a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_lod_reduce(const float4* seg, float4* qu_lod, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t D,
+                             hipStream_t s) {
+    const uint32_t nd = 1u << D;
+    const uint32_t by = ny / nd, bz = nz / nd;
+    (void)nx;
+    uint32_t P = 1;
+    while (P < by * bz) P <<= 1;
+    if ((size_t)P * sizeof(float4) > 64 * 1024) return hipErrorInvalidValue;
+    const uint32_t threads = P < 1024 ? (P < 64 ? 64 : P) : 1024;
+    hipLaunchKernelGGL(k_lod_reduce, dim3(nd * nd * nd), dim3(threads), (size_t)P * sizeof(float4), s, seg, qu_lod,
+                       ny, nd, by, bz, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float* p, uint64_t n, float v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_codec(int store, int dir, const void* in, void* out, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const dim3 g(blocks_for(n, 256)), b(256);
+    if (store == STORE_FP16S) hipLaunchKernelGGL(k_codec<F16S>, g, b, 0, s, dir, in, out, n);
+    else if (store == STORE_FP16C) hipLaunchKernelGGL(k_codec<F16C>, g, b, 0, s, dir, in, out, n);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace ion

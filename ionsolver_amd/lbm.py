@@ -1,0 +1,412 @@
+"""Python mirror of the reference's ``Lbm`` / ``LbmConfig`` API over the C ABI
+of ``libionsolver_hip.so`` (include/ionsolver.h).
+
+Names, defaults and argument meanings follow the reference's Rust host:
+``LbmConfig`` (src/lbm/mod.rs:45-135), ``Units`` (src/lbm/units.rs),
+``Lbm::{new, initialize, run, do_time_step, get_time_step}``
+(src/lbm/mod.rs:166-272, 492) and ``file::{read, write}`` (src/file.rs:12-30).
+The reference's ``bwrite!``/``bread!`` on the public domain buffers become
+``Lbm.write_field`` / ``Lbm.read_field``. Errors raise ``IonError`` carrying the
+library's message; there is no CPU fallback: without the built library every
+call fails.
+"""
+import ctypes
+import enum
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libionsolver_hip.so")
+
+
+class IonError(RuntimeError):
+    pass
+
+
+class VelocitySet(enum.IntEnum):  # src/lbm/types.rs:16-26
+    D2Q9 = 0
+    D3Q15 = 1
+    D3Q19 = 2
+    D3Q27 = 3
+
+    def get_set_values(self):  # types.rs:37-46 (dimensions, velocities, transfers)
+        return {0: (2, 9, 3), 1: (3, 15, 5), 2: (3, 19, 5), 3: (3, 27, 9)}[int(self)]
+
+
+class RelaxationTime(enum.IntEnum):  # types.rs:55-61
+    Srt = 0
+    Trt = 1
+
+
+class FloatType(enum.IntEnum):  # types.rs:75-83
+    FP16S = 0
+    FP16C = 1
+    FP32 = 2
+
+    def size_of(self):
+        return 4 if self == FloatType.FP32 else 2
+
+
+class Propellant(enum.IntEnum):  # units.rs:214-222
+    H = 0
+    He = 1
+    Ne = 2
+    Ar = 3
+    Kr = 4
+    Xe = 5
+
+
+class Field(enum.IntEnum):
+    FI = 0
+    RHO = 1
+    U = 2
+    FLAGS = 3
+    F = 4
+    E_STAT = 5
+    B_STAT = 6
+    E_DYN = 7
+    B_DYN = 8
+    FQI = 9
+    Q = 10
+    QU_LOD = 11
+    EI = 12
+
+
+class FieldMode(enum.IntEnum):
+    EXACT = 0
+    FAST = 1
+
+
+# --------------------------------------------------------------------------
+# ctypes structures (layout of include/ionsolver.h)
+# --------------------------------------------------------------------------
+class _Units(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_float), ("kg", ctypes.c_float), ("s", ctypes.c_float), ("a", ctypes.c_float),
+                ("k", ctypes.c_float), ("prop", ctypes.c_int32)]
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [
+        ("velocity_set", ctypes.c_int32), ("relaxation_time", ctypes.c_int32), ("float_type", ctypes.c_int32),
+        ("units", _Units),
+        ("n_x", ctypes.c_uint32), ("n_y", ctypes.c_uint32), ("n_z", ctypes.c_uint32),
+        ("d_x", ctypes.c_uint32), ("d_y", ctypes.c_uint32), ("d_z", ctypes.c_uint32),
+        ("nu", ctypes.c_float), ("f_x", ctypes.c_float), ("f_y", ctypes.c_float), ("f_z", ctypes.c_float),
+        ("ext_equilibrium_boudaries", ctypes.c_uint8), ("ext_volume_force", ctypes.c_uint8),
+        ("ext_force_field", ctypes.c_uint8), ("ext_magneto_hydro", ctypes.c_uint8),
+        ("ext_subgrid_ecr", ctypes.c_uint8), ("mhd_lod_depth", ctypes.c_uint8),
+        ("graphics_active", ctypes.c_uint8), ("reserved0", ctypes.c_uint8),
+        ("ecr_freq", ctypes.c_float), ("ecr_field_strength", ctypes.c_float),
+        ("run_steps", ctypes.c_uint64),
+    ]
+
+
+class _KernelConstants(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_float), ("c", ctypes.c_float), ("ke", ctypes.c_float), ("kmu", ctypes.c_float),
+                ("kmu0", ctypes.c_float), ("kkge", ctypes.c_float), ("wq", ctypes.c_float),
+                ("n_lod", ctypes.c_uint32)]
+
+
+class _Timing(ctypes.Structure):
+    _fields_ = [("ms_stream_collide", ctypes.c_double), ("ms_lod_reduce", ctypes.c_double),
+                ("ms_field_prep", ctypes.c_double), ("ms_update_e_b", ctypes.c_double),
+                ("ms_other", ctypes.c_double), ("n_steps", ctypes.c_uint64)]
+
+
+# Every entry point of include/ionsolver.h: name -> (restype, argtypes)
+_VP, _I, _U64, _F, _D, _SZ = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_float, ctypes.c_double, ctypes.c_size_t
+ABI = {
+    "ion_config_default": (None, [ctypes.POINTER(_Config)]),
+    "ion_units_default": (None, [ctypes.POINTER(_Units)]),
+    "ion_units_set": (None, [ctypes.POINTER(_Units)] + [_F] * 10),
+    "ion_kernel_constants_of": (_I, [ctypes.POINTER(_Config), ctypes.POINTER(_KernelConstants)]),
+    "ion_create": (_I, [ctypes.POINTER(_Config), ctypes.POINTER(_VP)]),
+    "ion_destroy": (None, [_VP]),
+    "ion_get_config": (_I, [_VP, ctypes.POINTER(_Config)]),
+    "ion_initialize": (_I, [_VP]),
+    "ion_run": (_I, [_VP, _U64]),
+    "ion_do_time_step": (_I, [_VP]),
+    "ion_get_time_step": (_U64, [_VP]),
+    "ion_update_fields": (_I, [_VP]),
+    "ion_synchronize": (_I, [_VP]),
+    "ion_field_size": (_I, [_VP, _I, ctypes.POINTER(_SZ), ctypes.POINTER(ctypes.c_uint32)]),
+    "ion_write_field": (_I, [_VP, _I, _VP, _SZ]),
+    "ion_read_field": (_I, [_VP, _I, _VP, _SZ]),
+    "ion_set_volume_force": (_I, [_VP, _F, _F, _F]),
+    "ion_save": (_I, [_VP, ctypes.c_char_p]),
+    "ion_load": (_I, [ctypes.c_char_p, ctypes.POINTER(_Config), ctypes.POINTER(_VP)]),
+    "ion_conservation": (_I, [_VP, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "ion_set_field_mode": (_I, [_VP, _I]),
+    "ion_set_timing": (_I, [_VP, _I]),
+    "ion_get_timing": (_I, [_VP, ctypes.POINTER(_Timing)]),
+    "ion_device_codec": (_I, [_I, _I, _VP, _VP, _U64]),
+    "ion_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def load_library():
+    """Load libionsolver_hip.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise IonError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in ABI.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise IonError(f"[{rc}] {load_library().ion_last_error().decode()}")
+
+
+# --------------------------------------------------------------------------
+# Config mirror
+# --------------------------------------------------------------------------
+class Units:
+    """Unit conversion factors (src/lbm/units.rs:13-59)."""
+
+    def __init__(self):
+        u = _Units()
+        load_library().ion_units_default(ctypes.byref(u))
+        self._u = u
+
+    def __getattr__(self, name):
+        if name in ("m", "kg", "s", "a", "k", "prop"):
+            return getattr(self.__dict__["_u"], name)
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        if name in ("m", "kg", "s", "a", "k", "prop"):
+            setattr(self._u, name, value)
+        else:
+            super().__setattr__(name, value)
+
+    def set(self, lbm_length, lbm_velocity, lbm_rho, lbm_charge, lbm_temp, si_length, si_velocity, si_rho,
+            si_charge, si_temp):
+        load_library().ion_units_set(ctypes.byref(self._u), lbm_length, lbm_velocity, lbm_rho, lbm_charge, lbm_temp,
+                                     si_length, si_velocity, si_rho, si_charge, si_temp)
+
+    # conversions used by setups (units.rs:62-147), f32 arithmetic
+    def nu_si_lu(self, nu):
+        f = np.float32
+        return float(f(nu) / (f(self.m) * f(self.m) / f(self.s)))
+
+    def e_field_si_lu(self, e):
+        f = np.float32
+        return float(f(e) / ((f(self.kg) * f(self.m)) / (f(self.a) * (f(self.s) * f(self.s) * f(self.s)))))
+
+    def mag_flux_si_lu(self, b):
+        f = np.float32
+        return float(f(b) / (f(self.kg) / (f(self.a) * (f(self.s) * f(self.s)))))
+
+
+class GraphicsConfig:
+    """Only ``graphics_active`` reaches the kernels (UPDATE_FIELDS, domain.rs:856)."""
+
+    def __init__(self):
+        self.graphics_active = True  # src/lbm/graphics.rs:160
+
+
+class LbmConfig:
+    """LbmConfig with the reference defaults (src/lbm/mod.rs:105-135)."""
+
+    def __init__(self):
+        self.velocity_set = VelocitySet.D2Q9
+        self.relaxation_time = RelaxationTime.Srt
+        self.float_type = FloatType.FP16S
+        self.units = Units()
+        self.n_x = self.n_y = self.n_z = 1
+        self.d_x = self.d_y = self.d_z = 1
+        self.nu = float(np.float32(1.0) / np.float32(6.0))
+        self.f_x = self.f_y = self.f_z = 0.0
+        self.ext_equilibrium_boudaries = False
+        self.ext_volume_force = False
+        self.ext_force_field = False
+        self.ext_magneto_hydro = False
+        self.ext_subgrid_ecr = False
+        self.mhd_lod_depth = 4
+        self.ecr_freq = 0.0
+        self.ecr_field_strength = 0.0
+        self.graphics_config = GraphicsConfig()
+        self.run_steps = 0
+
+    new = classmethod(lambda cls: cls())
+
+    def _to_c(self):
+        c = _Config()
+        c.velocity_set = int(self.velocity_set)
+        c.relaxation_time = int(self.relaxation_time)
+        c.float_type = int(self.float_type)
+        c.units = self.units._u
+        c.n_x, c.n_y, c.n_z = self.n_x, self.n_y, self.n_z
+        c.d_x, c.d_y, c.d_z = self.d_x, self.d_y, self.d_z
+        c.nu = self.nu
+        c.f_x, c.f_y, c.f_z = self.f_x, self.f_y, self.f_z
+        c.ext_equilibrium_boudaries = int(bool(self.ext_equilibrium_boudaries))
+        c.ext_volume_force = int(bool(self.ext_volume_force))
+        c.ext_force_field = int(bool(self.ext_force_field))
+        c.ext_magneto_hydro = int(bool(self.ext_magneto_hydro))
+        c.ext_subgrid_ecr = int(bool(self.ext_subgrid_ecr))
+        c.mhd_lod_depth = self.mhd_lod_depth
+        c.graphics_active = int(bool(self.graphics_config.graphics_active))
+        c.ecr_freq = self.ecr_freq
+        c.ecr_field_strength = self.ecr_field_strength
+        c.run_steps = self.run_steps
+        return c
+
+    @classmethod
+    def _from_c(cls, c):
+        cfg = cls()
+        cfg.velocity_set = VelocitySet(c.velocity_set)
+        cfg.relaxation_time = RelaxationTime(c.relaxation_time)
+        cfg.float_type = FloatType(c.float_type)
+        for k in ("m", "kg", "s", "a", "k", "prop"):
+            setattr(cfg.units, k, getattr(c.units, k))
+        cfg.n_x, cfg.n_y, cfg.n_z = c.n_x, c.n_y, c.n_z
+        cfg.d_x, cfg.d_y, cfg.d_z = c.d_x, c.d_y, c.d_z
+        cfg.nu = c.nu
+        cfg.f_x, cfg.f_y, cfg.f_z = c.f_x, c.f_y, c.f_z
+        cfg.ext_equilibrium_boudaries = bool(c.ext_equilibrium_boudaries)
+        cfg.ext_volume_force = bool(c.ext_volume_force)
+        cfg.ext_force_field = bool(c.ext_force_field)
+        cfg.ext_magneto_hydro = bool(c.ext_magneto_hydro)
+        cfg.ext_subgrid_ecr = bool(c.ext_subgrid_ecr)
+        cfg.mhd_lod_depth = c.mhd_lod_depth
+        cfg.graphics_config.graphics_active = bool(c.graphics_active)
+        cfg.ecr_freq, cfg.ecr_field_strength, cfg.run_steps = c.ecr_freq, c.ecr_field_strength, c.run_steps
+        return cfg
+
+    def kernel_constants(self):
+        """The f32 constants the reference bakes into its kernels (domain.rs:808, 836-849)."""
+        k = _KernelConstants()
+        _check(load_library().ion_kernel_constants_of(ctypes.byref(self._to_c()), ctypes.byref(k)))
+        return {f: getattr(k, f) for f, _ in k._fields_}
+
+
+_DTYPES = {4: np.float32, 2: np.uint16, 1: np.uint8}
+
+
+class Lbm:
+    """One simulation on one GPU (reference ``Lbm`` with a single domain)."""
+
+    def __init__(self, config, _handle=None):
+        self._lib = load_library()
+        if _handle is None:
+            h = ctypes.c_void_p()
+            _check(self._lib.ion_create(ctypes.byref(config._to_c()), ctypes.byref(h)))
+            _handle = h
+        self._h = _handle
+        c = _Config()
+        _check(self._lib.ion_get_config(self._h, ctypes.byref(c)))
+        self.config = LbmConfig._from_c(c)
+        self.n = self.config.n_x * self.config.n_y * self.config.n_z
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ion_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # Lbm::initialize / run / do_time_step / get_time_step
+    def initialize(self):
+        _check(self._lib.ion_initialize(self._h))
+
+    def run(self, steps):
+        _check(self._lib.ion_run(self._h, int(steps)))
+
+    def do_time_step(self):
+        _check(self._lib.ion_do_time_step(self._h))
+
+    def get_time_step(self):
+        return int(self._lib.ion_get_time_step(self._h))
+
+    def update_fields(self):
+        _check(self._lib.ion_update_fields(self._h))
+
+    def synchronize(self):
+        _check(self._lib.ion_synchronize(self._h))
+
+    # buffers
+    def field_size(self, field):
+        b = ctypes.c_size_t()
+        e = ctypes.c_uint32()
+        _check(self._lib.ion_field_size(self._h, int(field), ctypes.byref(b), ctypes.byref(e)))
+        return b.value, e.value
+
+    def read_field(self, field):
+        nbytes, esz = self.field_size(field)
+        if nbytes == 0:
+            raise IonError(f"field {Field(field).name} does not exist in this configuration")
+        out = np.empty(nbytes // esz, dtype=_DTYPES[esz])
+        _check(self._lib.ion_read_field(self._h, int(field), out.ctypes.data_as(ctypes.c_void_p), nbytes))
+        return out
+
+    def write_field(self, field, data):
+        nbytes, esz = self.field_size(field)
+        arr = np.ascontiguousarray(data, dtype=_DTYPES[esz])
+        _check(self._lib.ion_write_field(self._h, int(field), arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes))
+
+    def set_volume_force(self, fx, fy, fz):
+        _check(self._lib.ion_set_volume_force(self._h, fx, fy, fz))
+
+    # persistence (src/file.rs)
+    def save(self, path):
+        _check(self._lib.ion_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path, config=None):
+        c = (config or LbmConfig())._to_c()
+        h = ctypes.c_void_p()
+        _check(load_library().ion_load(os.fsencode(path), ctypes.byref(c), ctypes.byref(h)))
+        return cls(None, _handle=h)
+
+    # diagnostics / measurement
+    def conservation(self):
+        m = ctypes.c_double()
+        q = ctypes.c_double()
+        _check(self._lib.ion_conservation(self._h, ctypes.byref(m), ctypes.byref(q)))
+        return m.value, q.value
+
+    def set_field_mode(self, mode):
+        _check(self._lib.ion_set_field_mode(self._h, int(mode)))
+
+    def set_timing(self, enable):
+        _check(self._lib.ion_set_timing(self._h, int(bool(enable))))
+
+    def get_timing(self):
+        t = _Timing()
+        _check(self._lib.ion_get_timing(self._h, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in t._fields_}
+
+
+def device_codec(float_type, direction, data):
+    """Run the device storage codec on a host array (0: encode, 1: decode)."""
+    L = load_library()
+    if direction == 0:
+        src = np.ascontiguousarray(data, np.float32)
+        out = np.empty(src.shape, np.uint16)
+    else:
+        src = np.ascontiguousarray(data, np.uint16)
+        out = np.empty(src.shape, np.float32)
+    _check(L.ion_device_codec(int(float_type), direction, src.ctypes.data_as(ctypes.c_void_p),
+                              out.ctypes.data_as(ctypes.c_void_p), src.size))
+    return out

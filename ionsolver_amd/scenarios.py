@@ -1,0 +1,105 @@
+"""Synthetic workloads of the benchmark and the tests, as host-side setups.
+
+Each setup returns an ``LbmConfig`` plus the host arrays a caller writes into
+the domain buffers before ``initialize()``, the way the reference's setups do
+with ``bwrite!`` (src/setup.rs). Arrays are SoA in the reference layout
+(n = x + (y + z*Ny)*Nx, vector fields as [x-block, y-block, z-block]).
+"""
+import numpy as np
+
+from .lbm import FloatType, LbmConfig, VelocitySet
+
+# Units of the MHD workloads: the reference's setup_field_vis scaling
+# (src/setup.rs:244-246: 128 cells = 0.1 m, 1 m/s, air density) with the
+# charge unit chosen so that the Coulomb constant is ~1e-2 in lattice units.
+MHD_UNITS = dict(lbm_length=128.0, lbm_velocity=1.0, lbm_rho=1.0, lbm_charge=1.0, lbm_temp=1.0, si_length=0.1,
+                 si_velocity=1.0, si_rho=1.2250, si_charge=7e-13, si_temp=1.0)
+
+
+def cell_coords(nx, ny, nz):
+    z, y, x = np.meshgrid(np.arange(nz, dtype=np.float64), np.arange(ny, dtype=np.float64),
+                          np.arange(nx, dtype=np.float64), indexing="ij")
+    return x.ravel(), y.ravel(), z.ravel()
+
+
+def fluid_tgv(n=64, tau=0.6, u0=0.05, float_type=FloatType.FP32, velocity_set=VelocitySet.D3Q19):
+    """Fluid-only D3Q19 BGK Taylor-Green vortex in a periodic n^3 box.
+
+    Divergence-free 2D form u = u0 cos(kx) sin(ky), v = -u0 sin(kx) cos(ky),
+    w = 0, with the matching pressure rho = 1 - 3 u0^2/4 (cos 2kx + cos 2ky),
+    on cell-centred coordinates x + 0.5 - n/2 (src/setup.rs:512-514)."""
+    cfg = LbmConfig()
+    cfg.velocity_set = velocity_set
+    cfg.float_type = float_type
+    cfg.n_x = cfg.n_y = cfg.n_z = n
+    cfg.nu = float(np.float32((np.float32(tau) - np.float32(0.5)) / np.float32(3.0)))
+    x, y, z = cell_coords(n, n, n)
+    k = 2.0 * np.pi / n
+    fx, fy = x + 0.5 - 0.5 * n, y + 0.5 - 0.5 * n
+    ux = u0 * np.cos(k * fx) * np.sin(k * fy)
+    uy = -u0 * np.sin(k * fx) * np.cos(k * fy)
+    uz = np.zeros_like(ux)
+    rho = 1.0 - 0.75 * u0 * u0 * (np.cos(2 * k * fx) + np.cos(2 * k * fy))
+    u = np.concatenate([ux, uy, uz]).astype(np.float32)
+    return cfg, dict(rho=rho.astype(np.float32), u=u)
+
+
+def mhd_blobs(n=256, seed=1234, float_type=FloatType.FP32, lod_depth=4, n_blobs=8, sigma=6.0, amp=0.05,
+              b0=0.05, dipole=3200.0, cyl_r=40.0, nu=None, shape=None):
+    """D3Q19 BGK + MHD: Gaussian charge blobs, uniform B0 plus a point dipole,
+    Guo Lorentz forcing, a solid z-aligned cylinder (flags 0x01).
+
+    Blob centres are uniform in [2 sigma, n - 2 sigma)^3 outside the
+    cylinder; sign +-1 and amplitude U[0.5, 1]*amp come from the same seeded
+    stream. Returns (cfg, arrays, blobs)."""
+    nx, ny, nz = shape if shape is not None else (n, n, n)
+    cfg = LbmConfig()
+    cfg.velocity_set = VelocitySet.D3Q19
+    cfg.float_type = float_type
+    cfg.n_x, cfg.n_y, cfg.n_z = nx, ny, nz
+    cfg.units.set(**MHD_UNITS)
+    cfg.nu = float(np.float32(0.1) / np.float32(3.0)) if nu is None else nu
+    cfg.ext_volume_force = True
+    cfg.ext_magneto_hydro = True
+    cfg.mhd_lod_depth = lod_depth
+    rng = np.random.default_rng(seed)
+    x, y, z = cell_coords(nx, ny, nz)
+    cx, cy = 0.5 * nx, 0.5 * ny
+    r_axis = np.sqrt((x - cx) ** 2 + (y - cy) ** 2)
+    scale = nx / 256.0
+    solid = r_axis < cyl_r * scale
+    flags = np.where(solid, 1, 0).astype(np.uint8)
+    blobs = []
+    lo = 2 * sigma
+    while len(blobs) < n_blobs:
+        c = rng.uniform([lo, lo, lo], [nx - lo, ny - lo, nz - lo])
+        if np.hypot(c[0] - cx, c[1] - cy) < cyl_r * scale + 2 * sigma:
+            continue
+        sign = 1.0 if rng.uniform() < 0.5 else -1.0
+        a = sign * amp * rng.uniform(0.5, 1.0)
+        blobs.append((float(c[0]), float(c[1]), float(c[2]), float(a)))
+    q = np.zeros(nx * ny * nz)
+    for bx, by, bz, a in blobs:
+        q += a * np.exp(-((x - bx) ** 2 + (y - by) ** 2 + (z - bz) ** 2) / (2 * sigma * sigma))
+    q[solid] = 0.0
+    # B_stat = B0 z + point dipole m z at the domain centre: (3 (m.r^) r^ - m) / |r|^3
+    rx, ry, rz = x - cx, y - cy, z - 0.5 * nz
+    r2 = rx * rx + ry * ry + rz * rz
+    r2[r2 == 0] = np.inf
+    r = np.sqrt(r2)
+    mdotr = dipole * scale ** 3 * rz / r
+    bxf = (3 * mdotr * rx / r) / r2 / r
+    byf = (3 * mdotr * ry / r) / r2 / r
+    bzf = (3 * mdotr * rz / r - dipole * scale ** 3) / r2 / r + b0
+    b_stat = np.concatenate([bxf, byf, bzf]).astype(np.float32)
+    arrays = dict(flags=flags, q=q.astype(np.float32), b_stat=b_stat)
+    return cfg, arrays, blobs
+
+
+def apply(lbm, arrays):
+    """Write setup arrays into the domain buffers (the setups' bwrite! calls)."""
+    from .lbm import Field
+    names = dict(flags=Field.FLAGS, rho=Field.RHO, u=Field.U, q=Field.Q, e_stat=Field.E_STAT, b_stat=Field.B_STAT,
+                 F=Field.F)
+    for k, v in arrays.items():
+        lbm.write_field(names[k], v)
