@@ -101,9 +101,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="mhd256")
-    ap.add_argument("--field-mode", default="exact", choices=["exact", "fast"])
+    ap.add_argument("--field-mode", default="fast", choices=["exact", "fast"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run the device FP16C decode on 2^26 codes (reads 128 MiB, writes 256 MiB): "
+                         "the reference streaming kernel for PMC FETCH_SIZE/WRITE_SIZE scale factors")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
@@ -113,6 +116,8 @@ def main():
     import ionsolver_amd as I
     from ionsolver_amd import FieldMode, scenarios
 
+    if args.calibrate:
+        I.device_codec(I.FloatType.FP16C, 1, np.zeros(1 << 26, np.uint16))
     cfg, arrays, meta = workload(args.workload)
     lbm = I.Lbm(cfg)
     scenarios.apply(lbm, arrays)

@@ -178,6 +178,84 @@ __global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b(FieldArgs a) {
     }
 }
 
+// FAST field arithmetic on LOD-aligned tiles. A block covers
+// (4*TX) x TY x TZ cells inside one top-level LOD block, so the own-LOD
+// skip is the same for every cell of the block. Each thread owns 4
+// consecutive cells of one row: the source terms that depend only on y and z
+// (r_y, r_z, r_y^2 + r_z^2 and the x component of v x r) are computed once per
+// source and shared by the 4 cells.
+constexpr int ROW = 4;
+
+__global__ void __launch_bounds__(256) k_update_e_b_fast_tiled(FieldArgs a, uint32_t TX, uint32_t TY, uint32_t TZ) {
+    __shared__ float4 s_pq[FIELD_TILE];
+    __shared__ float4 s_vd[FIELD_TILE];
+    const uint32_t NX = a.nx, NY = a.ny;
+    const uint32_t tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);
+    const uint32_t tiles_x = NX / (ROW * TX), tiles_y = NY / TY;
+    const uint32_t b = blockIdx.x;
+    const uint32_t bx = b % tiles_x, by = (b / tiles_x) % tiles_y, bz = b / (tiles_x * tiles_y);
+    const uint32_t x0 = bx * ROW * TX + ROW * tx, y = by * TY + ty, z = bz * TZ + tz;
+    const uint32_t nd = 1u << a.D;
+    const uint32_t own = (bx * ROW * TX) / (NX / nd) + ((by * TY) / (NY / nd) + (bz * TZ) / (a.nz / nd) * nd) * nd;
+    const float py = (float)y, pz = (float)z;
+    float px[ROW], ex[ROW], ey[ROW], ez[ROW], bxa[ROW], bya[ROW], bza[ROW];
+#pragma unroll
+    for (int k = 0; k < ROW; k++) {
+        px[k] = (float)(x0 + k);
+        ex[k] = ey[k] = ez[k] = bxa[k] = bya[k] = bza[k] = 0.0f;
+    }
+    const uint32_t count = *a.src_count;
+    for (uint32_t base = 0; base < count; base += FIELD_TILE) {
+        const uint32_t m = min(FIELD_TILE, count - base);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+            s_pq[i] = a.src_pq[base + i];
+            s_vd[i] = a.src_vd[base + i];
+        }
+        __syncthreads();
+        for (uint32_t s = 0; s < m; s++) {
+            const float4 vd = s_vd[s];
+            if (__float_as_uint(vd.w) == own) continue;  // uniform over the block
+            const float4 pq = s_pq[s];
+            const float ry = py - pq.y, rz = pz - pq.z;
+            const float syz = __builtin_fmaf(ry, ry, rz * rz);
+            const float cx = __builtin_fmaf(vd.y, rz, -(vd.z * ry));  // (v x r).x
+            const float m1 = vd.x * rz, m2 = vd.x * ry;
+#pragma unroll
+            for (int k = 0; k < ROW; k++) {
+                const float rx = px[k] - pq.x;
+                const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, syz));
+                const float t = pq.w * (inv * inv * inv);
+                ex[k] = __builtin_fmaf(t, rx, ex[k]);
+                ey[k] = __builtin_fmaf(t, ry, ey[k]);
+                ez[k] = __builtin_fmaf(t, rz, ez[k]);
+                bxa[k] = __builtin_fmaf(t, cx, bxa[k]);
+                bya[k] = __builtin_fmaf(t, __builtin_fmaf(vd.z, rx, -m1), bya[k]);  // (v x r).y
+                bza[k] = __builtin_fmaf(t, __builtin_fmaf(-vd.y, rx, m2), bza[k]);  // (v x r).z
+            }
+        }
+    }
+    const uint64_t N = a.N;
+#pragma unroll
+    for (int k = 0; k < ROW; k++) {
+        const uint32_t n = (x0 + k) + (y + z * NY) * NX;
+        if ((a.flags[n] & 0x1F) == 0x01) continue;
+        a.e_dyn[n] = a.e_stat[n] + a.ke * ex[k];
+        a.e_dyn[N + n] = a.e_stat[N + n] + a.ke * ey[k];
+        a.e_dyn[2 * N + n] = a.e_stat[2 * N + n] + a.ke * ez[k];
+        a.b_dyn[n] = a.b_stat[n] + a.kmu * bxa[k];
+        a.b_dyn[N + n] = a.b_stat[N + n] + a.kmu * bya[k];
+        a.b_dyn[2 * N + n] = a.b_stat[2 * N + n] + a.kmu * bza[k];
+    }
+}
+
+// Largest power of two <= cap dividing v.
+static uint32_t pow2_div(uint32_t v, uint32_t cap) {
+    uint32_t p = 1;
+    while (p * 2 <= cap && v % (p * 2) == 0) p *= 2;
+    return p;
+}
+
 // Float64 sum of x over fluid cells (flags & 0x1F != 0x01): one partial per
 // block, summed on the host in block order.
 __global__ void __launch_bounds__(256) k_sum_f64(const float* __restrict__ x, const uint8_t* __restrict__ flags,
@@ -205,6 +283,21 @@ hipError_t launch_prep_sources(const float4* qu_lod, uint32_t lo, uint32_t hi, u
 
 hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s) {
     constexpr int CPT_EXACT = 1, CPT_FAST = 2;
+    if (fast && !a.near && a.D > 0) {
+        // LOD-aligned tiles: (ROW*TX) | lod block x, TY | lod block y, TZ | lod block z
+        const uint32_t nd = 1u << a.D;
+        const uint32_t lbx = a.nx / nd, lby = a.ny / nd, lbz = a.nz / nd;
+        if (lbx % ROW == 0) {
+            const uint32_t TX = pow2_div(lbx / ROW, 16);
+            const uint32_t TY = pow2_div(lby, 256 / TX);
+            const uint32_t TZ = pow2_div(lbz, 256 / (TX * TY));
+            if (TX * TY * TZ >= 64) {
+                const uint32_t blocks = (a.nx / (ROW * TX)) * (a.ny / TY) * (a.nz / TZ);
+                hipLaunchKernelGGL(k_update_e_b_fast_tiled, dim3(blocks), dim3(TX * TY * TZ), 0, s, a, TX, TY, TZ);
+                return hipGetLastError();
+            }
+        }
+    }
     const int cpt = fast ? CPT_FAST : CPT_EXACT;
     const uint64_t per = (uint64_t)FIELD_BLOCK * cpt;
     const dim3 g((uint32_t)((a.N + per - 1) / per)), b(FIELD_BLOCK);

@@ -62,6 +62,27 @@ __device__ __forceinline__ void load_ddfs(const typename S::T* __restrict__ buf,
         fh[i + 1] = S::ld(buf[sn * N + j]);
     });
 }
+// Raw (undecoded) esoteric-pull loads, so that a kernel can issue the loads
+// of all its lattices before the first decode and keep them in flight together.
+template <int Q, int NQ, class S>
+__device__ __forceinline__ void load_raw(const typename S::T* __restrict__ buf, uint64_t N, uint32_t n, const Nbr& b,
+                                         uint32_t odd, typename S::T* raw) {
+    raw[0] = buf[n];
+    static_for<(NQ - 1) / 2>([&](auto p) {
+        constexpr int i = 2 * decltype(p)::value + 1;
+        const uint32_t j = nb_index<Q, i>(b);
+        const uint64_t so = odd ? (uint64_t)i : (uint64_t)(i + 1);
+        const uint64_t sn = odd ? (uint64_t)(i + 1) : (uint64_t)i;
+        raw[i] = __builtin_nontemporal_load(&buf[so * N + n]);
+        raw[i + 1] = __builtin_nontemporal_load(&buf[sn * N + j]);
+    });
+}
+template <int NQ, class S>
+__device__ __forceinline__ void decode(const typename S::T* raw, float* fh) {
+#pragma unroll
+    for (int i = 0; i < NQ; i++) fh[i] = S::ld(raw[i]);
+}
+
 template <int Q, int NQ, class S>
 __device__ __forceinline__ void store_ddfs(typename S::T* __restrict__ buf, uint64_t N, uint32_t n, const Nbr& b,
                                            uint32_t odd, const float* fh) {
@@ -71,8 +92,8 @@ __device__ __forceinline__ void store_ddfs(typename S::T* __restrict__ buf, uint
         const uint32_t j = nb_index<Q, i>(b);
         const uint64_t so = odd ? (uint64_t)i : (uint64_t)(i + 1);
         const uint64_t sn = odd ? (uint64_t)(i + 1) : (uint64_t)i;
-        buf[sn * N + j] = S::st(fh[i]);
-        buf[so * N + n] = S::st(fh[i + 1]);
+        __builtin_nontemporal_store(S::st(fh[i]), &buf[sn * N + j]);
+        __builtin_nontemporal_store(S::st(fh[i + 1]), &buf[so * N + n]);
     });
 }
 
@@ -134,8 +155,18 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
     if (active) {
         const bool is_e = a.eqb && bo == 0x02;
         const Nbr b = neighbours(n, a.nx, a.ny, a.nz);
+        // every load of the cell first: fi, ei, fqi, E_dyn, B_dyn
+        T rf[Q], re[MHD ? Q : 1], rq[MHD ? 7 : 1];
+        float Bx = 0.0f, By = 0.0f, Bz = 0.0f, Ex = 0.0f, Ey = 0.0f, Ez = 0.0f;
+        load_raw<Q, Q, S>((const T*)a.fi, N, n, b, a.odd, rf);
+        if (MHD) {
+            load_raw<Q, Q, S>((const T*)a.ei, N, n, b, a.odd, re);
+            load_raw<19, 7, S>((const T*)a.fqi, N, n, b, a.odd, rq);
+            Bx = a.b_dyn[n]; By = a.b_dyn[N + n]; Bz = a.b_dyn[2 * N + n];
+            Ex = a.e_dyn[n]; Ey = a.e_dyn[N + n]; Ez = a.e_dyn[2 * N + n];
+        }
         float fhn[Q];
-        load_ddfs<Q, Q, S>((const T*)a.fi, N, n, b, a.odd, fhn);
+        decode<Q, S>(rf, fhn);
         float rhon, uxn, uyn, uzn;
         if (is_e) {
             rhon = a.rho[n];
@@ -155,16 +186,14 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
             fzn += a.F[2 * N + n];
         }
         if (MHD) {
-            const float Bx = a.b_dyn[n], By = a.b_dyn[N + n], Bz = a.b_dyn[2 * N + n];
-            const float Ex = a.e_dyn[n], Ey = a.e_dyn[N + n], Ez = a.e_dyn[2 * N + n];
             // electron gas part 1
             float ehn[Q];
-            load_ddfs<Q, Q, S>((const T*)a.ei, N, n, b, a.odd, ehn);
+            decode<Q, S>(re, ehn);
             float rhon_e, uxn_e, uyn_e, uzn_e;
             rho_u<Q>(ehn, rhon_e, uxn_e, uyn_e, uzn_e);
             // charge advection part 1 (D3Q7 subset of D3Q19 neighbours)
             float qhn[7];
-            load_ddfs<19, 7, S>((const T*)a.fqi, N, n, b, a.odd, qhn);
+            decode<7, S>(rq, qhn);
             float rhon_q = 0.0f;
 #pragma unroll
             for (int i = 0; i < 7; i++) rhon_q += qhn[i];
