@@ -89,7 +89,11 @@ typedef enum ion_field {
     ION_FIELD_Q = 10,
     ION_FIELD_QU_LOD = 11,
     ION_FIELD_EI = 12,     /* electron-gas DDFs */
-    ION_FIELD_COUNT = 13
+    /* no reference counterpart: the raw source sums of the last direct field
+     * update (E then B, 6 x N f32, before the KE / KMU scaling), kept once a
+     * surrogate exists (the surrogate's training and judging reference) */
+    ION_FIELD_FIELD_SUMS = 13,
+    ION_FIELD_COUNT = 14
 } ion_field;
 
 /* The f32 kernel constants the reference bakes in as #defines
@@ -181,6 +185,62 @@ int ion_get_timing(ion_handle* h, ion_timing* out);
 /* Device storage codec on host arrays (diagnostics): dir 0 encodes
  * float -> u16 code, dir 1 decodes u16 -> float, for ION_FP16S/ION_FP16C. */
 int ion_device_codec(int float_type, int dir, const void* src, void* dst, uint64_t n);
+
+/* ------------------------------------------------------------------------
+ * Learned field surrogate (build-side; the reference has no counterpart).
+ * A linear 4-level 3D conv neural operator on the LOD grid (BF16 MFMA, FP32
+ * accumulation) that maps the top-level LOD charge and current sums to
+ * E - E_stat and B - B_stat at every cell. The direct update_e_b_dynamic
+ * is its judge: with ION_SOLVER_SURROGATE the direct kernel runs every
+ * `judge_every` field steps, the relative L2 residual of the surrogate is
+ * measured against it, and a residual above `tolerance` switches the next
+ * `judge_every` field steps back to the direct kernel (a fallback).
+ * Needs ext_magneto_hydro, mhd_lod_depth >= 3 and an empty near-field block.
+ * ------------------------------------------------------------------------ */
+typedef enum { ION_SOLVER_DIRECT = 0, ION_SOLVER_SURROGATE = 1 } ion_field_solver;
+
+typedef struct ion_surrogate_config {
+    int32_t channels;  /* level-0 channels c0; levels use c0, 2c0, 3c0, 4c0 (default 16) */
+    uint64_t seed;     /* weight initialisation */
+    float lr;          /* Adam step of the U-Net layers and the head's feature columns (default 1e-3) */
+    float lr_head;     /* Adam step of the head's direct-source (near field) columns (default 1e-4) */
+} ion_surrogate_config;
+
+typedef struct ion_surrogate_stats {
+    uint64_t train_steps;
+    uint64_t direct_steps;     /* field steps solved by the direct kernel */
+    uint64_t surrogate_steps;  /* field steps solved by the surrogate */
+    uint64_t fallbacks;        /* judge residuals above the tolerance */
+    double last_residual_e;    /* relative L2 error of the dynamic part E - E_stat at the last judge step */
+    double last_residual_b;
+} ion_surrogate_stats;
+
+void ion_surrogate_config_default(ion_surrogate_config* cfg);
+int ion_surrogate_create(ion_handle* h, const ion_surrogate_config* cfg);
+/* LbmDomain::enqueue_update_e_b_dyn (src/lbm/domain.rs:443): direct field
+ * update from the current LOD sums, Q and u. Once a surrogate exists it also
+ * keeps the raw source sums (the dynamic parts before the KE / KMU scaling),
+ * which are the surrogate's training and judging reference; E_dyn / B_dyn
+ * are bit-identical either way. */
+int ion_compute_fields(ion_handle* h);
+/* Surrogate field from the current LOD sums into E_dyn / B_dyn. */
+int ion_surrogate_infer(ion_handle* h);
+/* One training step against the raw sums of the last ion_compute_fields
+ * (call it first); update = 0 computes gradients only. rel_e / rel_b:
+ * relative L2 errors of the dynamic parts before the step. */
+int ion_surrogate_train_step(ion_handle* h, int update, double* rel_e, double* rel_b);
+/* Relative L2 errors of the surrogate's dynamic E and B against the raw sums
+ * of the last ion_compute_fields. */
+int ion_surrogate_residual(ion_handle* h, double* rel_e, double* rel_b);
+/* Adam steps for the following train steps (schedules). */
+int ion_surrogate_set_learning_rate(ion_handle* h, float lr, float lr_head);
+int ion_set_field_solver(ion_handle* h, int solver, uint32_t judge_every, double tolerance);
+int ion_surrogate_get_stats(ion_handle* h, ion_surrogate_stats* out);
+/* Flat fp32 parameters (or gradients of the last train step): layers
+ * e0, e1, e2, e3, d2, d1, d0, head, each [cout][kpad] row-major. */
+int ion_surrogate_param_count(ion_handle* h, uint64_t* n);
+int ion_surrogate_read_params(ion_handle* h, int grads, float* dst, uint64_t n);
+int ion_surrogate_write_params(ion_handle* h, const float* src, uint64_t n);
 
 /* Message of the last failure on this thread ("" if none). */
 const char* ion_last_error(void);

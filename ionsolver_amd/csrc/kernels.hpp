@@ -56,6 +56,7 @@ struct FieldArgs {
     uint32_t dsx, dsy, dsz; // near-field block side
     uint32_t near;          // near-field block has more than one cell
     float ke, kmu;
+    float* raw;             // if set: the raw source sums (E then B, 6 x N) of fluid cells instead of E_dyn / B_dyn
 };
 
 // kernels_lbm.hip
@@ -74,6 +75,9 @@ hipError_t launch_prep_sources(const float4* qu_lod, uint32_t lo, uint32_t hi, u
                                uint32_t nz, uint32_t D, float4* src_pq, float4* src_vd, uint32_t* count,
                                hipStream_t s);
 hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s);
+// E_dyn = E_stat + KE raw_E, B_dyn = B_stat + KMU raw_B for fluid cells (the
+// direct kernel's own epilogue, applied to a.raw).
+hipError_t launch_combine_fields(const FieldArgs& a, hipStream_t s);
 hipError_t launch_sum_f64(const float* x, const uint8_t* flags, uint64_t n, double* partial, uint32_t nblocks,
                           hipStream_t s);
 

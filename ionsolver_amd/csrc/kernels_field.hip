@@ -169,6 +169,16 @@ __global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b(FieldArgs a) {
     for (int k = 0; k < CPT; k++) {
         if (!act[k]) continue;
         const uint32_t n = cell[k];
+        if (a.raw != nullptr) {
+            float* r = a.raw;
+            r[n] = ex[k];
+            r[N + n] = ey[k];
+            r[2 * N + n] = ez[k];
+            r[3 * N + n] = bx[k];
+            r[4 * N + n] = by[k];
+            r[5 * N + n] = bz[k];
+            continue;
+        }
         a.e_dyn[n] = a.e_stat[n] + a.ke * ex[k];
         a.e_dyn[N + n] = a.e_stat[N + n] + a.ke * ey[k];
         a.e_dyn[2 * N + n] = a.e_stat[2 * N + n] + a.ke * ez[k];
@@ -240,6 +250,16 @@ __global__ void __launch_bounds__(256) k_update_e_b_fast_tiled(FieldArgs a, uint
     for (int k = 0; k < ROW; k++) {
         const uint32_t n = (x0 + k) + (y + z * NY) * NX;
         if ((a.flags[n] & 0x1F) == 0x01) continue;
+        if (a.raw != nullptr) {
+            float* r = a.raw;
+            r[n] = ex[k];
+            r[N + n] = ey[k];
+            r[2 * N + n] = ez[k];
+            r[3 * N + n] = bxa[k];
+            r[4 * N + n] = bya[k];
+            r[5 * N + n] = bza[k];
+            continue;
+        }
         a.e_dyn[n] = a.e_stat[n] + a.ke * ex[k];
         a.e_dyn[N + n] = a.e_stat[N + n] + a.ke * ey[k];
         a.e_dyn[2 * N + n] = a.e_stat[2 * N + n] + a.ke * ez[k];
@@ -247,6 +267,19 @@ __global__ void __launch_bounds__(256) k_update_e_b_fast_tiled(FieldArgs a, uint
         a.b_dyn[N + n] = a.b_stat[N + n] + a.kmu * bya[k];
         a.b_dyn[2 * N + n] = a.b_stat[2 * N + n] + a.kmu * bza[k];
     }
+}
+
+__global__ void k_combine_fields(FieldArgs a) {
+    const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t N = a.N;
+    if (n >= N || (a.flags[n] & 0x1F) == 0x01) return;
+    const float* r = a.raw;
+    a.e_dyn[n] = a.e_stat[n] + a.ke * r[n];
+    a.e_dyn[N + n] = a.e_stat[N + n] + a.ke * r[N + n];
+    a.e_dyn[2 * N + n] = a.e_stat[2 * N + n] + a.ke * r[2 * N + n];
+    a.b_dyn[n] = a.b_stat[n] + a.kmu * r[3 * N + n];
+    a.b_dyn[N + n] = a.b_stat[N + n] + a.kmu * r[4 * N + n];
+    a.b_dyn[2 * N + n] = a.b_stat[2 * N + n] + a.kmu * r[5 * N + n];
 }
 
 // Largest power of two <= cap dividing v.
@@ -308,6 +341,12 @@ hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s) {
         if (a.near) hipLaunchKernelGGL((k_update_e_b<false, true, CPT_EXACT>), g, b, 0, s, a);
         else hipLaunchKernelGGL((k_update_e_b<false, false, CPT_EXACT>), g, b, 0, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_combine_fields(const FieldArgs& a, hipStream_t s) {
+    if (a.raw == nullptr) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_combine_fields, dim3((uint32_t)((a.N + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -16,6 +16,7 @@
 
 #include "ionsolver.h"
 #include "kernels.hpp"
+#include "surrogate_host.hpp"
 
 namespace {
 
@@ -93,6 +94,15 @@ struct ion_handle {
     std::vector<hipEvent_t> ev_pool;
     struct Pend { hipEvent_t a, b; int slot; };
     std::vector<Pend> pending;
+    // learned field surrogate and its judge
+    ion::sur::Net* net = nullptr;
+    float* field_raw = nullptr;  // raw direct source sums (E then B, 6 x N) once a surrogate exists
+    int solver = ION_SOLVER_DIRECT;
+    uint32_t judge_every = 50;
+    double tolerance = 0.01;
+    uint64_t field_steps = 0;
+    uint32_t fallback_left = 0;
+    ion_surrogate_stats sstats{};
 
     size_t ddf_elem() const { return store == ion::STORE_FP32 ? 4 : 2; }
 };
@@ -111,6 +121,8 @@ int dalloc(ion_handle* h, void** p, size_t bytes) {
 
 void free_all(ion_handle* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h->net;
+    h->net = nullptr;
     for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
@@ -240,7 +252,47 @@ int field_update(ion_handle* h) {
     f.near = ((uint64_t)f.dsx * f.dsy * f.dsz > 1) ? 1u : 0u;
     f.ke = h->kc.ke;
     f.kmu = h->kc.kmu;
-    return timed(h, 3, [&] { return ion::launch_update_e_b(f, h->field_mode == ION_FIELD_FAST, h->stream); });
+    f.raw = h->field_raw;
+    return timed(h, 3, [&] {
+        hipError_t e = ion::launch_update_e_b(f, h->field_mode == ION_FIELD_FAST, h->stream);
+        if (e == hipSuccess && f.raw != nullptr) e = ion::launch_combine_fields(f, h->stream);
+        return e;
+    });
+}
+
+ion::sur::Fields sur_fields(ion_handle* h) {
+    return ion::sur::Fields{(const float4*)h->qu_lod, h->flags, h->e_stat, h->b_stat, h->e_dyn, h->b_dyn, h->field_raw};
+}
+
+// The field step of a time step: the direct kernel, or the surrogate with the
+// direct kernel as its judge every judge_every field steps (and for
+// judge_every steps after a residual above the tolerance).
+int field_step(ion_handle* h) {
+    if (h->solver != ION_SOLVER_SURROGATE || !h->net) {
+        h->sstats.direct_steps += 1;
+        return field_update(h);
+    }
+    const bool judge = h->fallback_left > 0 || h->field_steps % h->judge_every == 0;
+    h->field_steps += 1;
+    int rc;
+    if (judge) {
+        if ((rc = field_update(h))) return rc;
+        h->sstats.direct_steps += 1;
+        if (h->fallback_left > 0) h->fallback_left -= 1;
+        double re = 0, rb = 0;
+        if (h->net->residual(sur_fields(h), &re, &rb)) return fail(ION_ERR_DEVICE, h->net->error());
+        h->sstats.last_residual_e = re;
+        h->sstats.last_residual_b = rb;
+        if (!(re <= h->tolerance && rb <= h->tolerance)) {  // a NaN residual is a miss too
+            h->sstats.fallbacks += 1;
+            h->fallback_left = h->judge_every;
+        }
+        return ION_OK;
+    }
+    rc = timed(h, 3, [&] { return h->net->infer(sur_fields(h)) ? hipErrorUnknown : hipSuccess; });
+    if (rc) return fail(ION_ERR_DEVICE, "surrogate: " + h->net->error());
+    h->sstats.surrogate_steps += 1;
+    return ION_OK;
 }
 
 int do_step(ion_handle* h) {
@@ -263,7 +315,7 @@ int do_step(ion_handle* h) {
             });
             if (rc) return rc;
         }
-        rc = field_update(h);
+        rc = field_step(h);
         if (rc) return rc;
     }
     h->t += 1;
@@ -505,6 +557,7 @@ static void* field_ptr(const ion_handle* h, int field, size_t* bytes, uint32_t* 
         case ION_FIELD_Q: p = h->q; b = p ? N * 4 : 0; break;
         case ION_FIELD_QU_LOD: p = h->qu_lod; b = p ? h->n_lod * 16 : 0; break;
         case ION_FIELD_EI: p = h->ei; b = p ? N * h->Q * de : 0; e = (uint32_t)de; break;
+        case ION_FIELD_FIELD_SUMS: p = h->field_raw; b = p ? 6 * N * 4 : 0; break;
         default: b = 0;
     }
     if (bytes) *bytes = b;
@@ -591,6 +644,141 @@ int ion_get_timing(ion_handle* h, ion_timing* out) {
     int rc = drain_timing(h);
     if (rc) return rc;
     *out = h->tm;
+    return ION_OK;
+}
+
+void ion_surrogate_config_default(ion_surrogate_config* c) {
+    c->channels = 16;
+    c->seed = 1;
+    c->lr = 1e-3f;
+    c->lr_head = 1e-4f;
+}
+
+int ion_surrogate_create(ion_handle* h, const ion_surrogate_config* cfg) {
+    if (!h || !cfg) return fail(ION_ERR_INVALID, "null argument");
+    if (!h->mhd) return fail(ION_ERR_INVALID, "the surrogate needs ext_magneto_hydro");
+    if (h->D < 3) return fail(ION_ERR_UNSUPPORTED, "the surrogate needs mhd_lod_depth >= 3");
+    const uint32_t sh = 1u << (h->nd & 31u);
+    if (h->cfg.n_x / sh > 1 || h->cfg.n_y / sh > 1 || h->cfg.n_z / sh > 1)
+        return fail(ION_ERR_UNSUPPORTED, "the surrogate needs an empty near-field block");
+    if (cfg->channels < 8 || cfg->channels % 8 != 0) return fail(ION_ERR_INVALID, "channels must be a positive multiple of 8");
+    delete h->net;
+    h->net = new ion::sur::Net();
+    ion::sur::Geom g{h->cfg.n_x, h->cfg.n_y, h->cfg.n_z, h->D, h->N,
+                     (uint32_t)(h->n_lod - (uint64_t)h->nd * h->nd * h->nd), h->kc.ke, h->kc.kmu};
+    ion::sur::Config c;
+    c.c0 = cfg->channels;
+    c.seed = cfg->seed;
+    c.lr = cfg->lr;
+    c.lr_head = cfg->lr_head;
+    const int rc = h->net->create(g, c, h->stream);
+    if (rc) {
+        std::string e = h->net->error();
+        delete h->net;
+        h->net = nullptr;
+        return fail(rc == -2 ? ION_ERR_UNSUPPORTED : ION_ERR_DEVICE, "surrogate: " + e);
+    }
+    if (!h->field_raw) {
+        const int r2 = dalloc(h, (void**)&h->field_raw, (size_t)6 * h->N * sizeof(float));
+        if (r2) return r2;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return ION_OK;
+}
+
+int ion_compute_fields(ion_handle* h) {
+    if (!h) return fail(ION_ERR_INVALID, "null handle");
+    if (!h->mhd) return fail(ION_ERR_INVALID, "fields need ext_magneto_hydro");
+    int rc = field_update(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return ION_OK;
+}
+
+static int need_net(ion_handle* h) {
+    if (!h) return fail(ION_ERR_INVALID, "null handle");
+    if (!h->net) return fail(ION_ERR_INVALID, "no surrogate: call ion_surrogate_create first");
+    return ION_OK;
+}
+
+int ion_surrogate_infer(ion_handle* h) {
+    int rc = need_net(h);
+    if (rc) return rc;
+    if (h->net->infer(sur_fields(h))) return fail(ION_ERR_DEVICE, "surrogate: " + h->net->error());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return ION_OK;
+}
+
+int ion_surrogate_train_step(ion_handle* h, int update, double* rel_e, double* rel_b) {
+    int rc = need_net(h);
+    if (rc) return rc;
+    double re = 0, rb = 0;
+    if (h->net->train_step(sur_fields(h), update != 0, &re, &rb)) return fail(ION_ERR_DEVICE, "surrogate: " + h->net->error());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (update) h->sstats.train_steps += 1;
+    if (rel_e) *rel_e = re;
+    if (rel_b) *rel_b = rb;
+    return ION_OK;
+}
+
+int ion_surrogate_residual(ion_handle* h, double* rel_e, double* rel_b) {
+    int rc = need_net(h);
+    if (rc) return rc;
+    double re = 0, rb = 0;
+    if (h->net->residual(sur_fields(h), &re, &rb)) return fail(ION_ERR_DEVICE, "surrogate: " + h->net->error());
+    if (rel_e) *rel_e = re;
+    if (rel_b) *rel_b = rb;
+    return ION_OK;
+}
+
+int ion_set_field_solver(ion_handle* h, int solver, uint32_t judge_every, double tolerance) {
+    if (!h) return fail(ION_ERR_INVALID, "null handle");
+    if (solver != ION_SOLVER_DIRECT && solver != ION_SOLVER_SURROGATE) return fail(ION_ERR_INVALID, "invalid solver");
+    if (solver == ION_SOLVER_SURROGATE && !h->net) return fail(ION_ERR_INVALID, "no surrogate: call ion_surrogate_create first");
+    if (judge_every == 0) return fail(ION_ERR_INVALID, "judge_every must be >= 1");
+    h->solver = solver;
+    h->judge_every = judge_every;
+    h->tolerance = tolerance;
+    h->field_steps = 0;
+    h->fallback_left = 0;
+    return ION_OK;
+}
+
+int ion_surrogate_set_learning_rate(ion_handle* h, float lr, float lr_head) {
+    int rc = need_net(h);
+    if (rc) return rc;
+    if (!(lr >= 0.0f) || !(lr_head >= 0.0f)) return fail(ION_ERR_INVALID, "learning rates must be >= 0");
+    h->net->set_learning_rate(lr, lr_head);
+    return ION_OK;
+}
+
+int ion_surrogate_get_stats(ion_handle* h, ion_surrogate_stats* out) {
+    if (!h || !out) return fail(ION_ERR_INVALID, "null argument");
+    *out = h->sstats;
+    return ION_OK;
+}
+
+int ion_surrogate_param_count(ion_handle* h, uint64_t* n) {
+    int rc = need_net(h);
+    if (rc) return rc;
+    *n = h->net->param_count();
+    return ION_OK;
+}
+
+int ion_surrogate_read_params(ion_handle* h, int grads, float* dst, uint64_t n) {
+    int rc = need_net(h);
+    if (rc) return rc;
+    if (!dst || n != h->net->param_count()) return fail(ION_ERR_INVALID, "parameter count mismatch");
+    if (h->net->copy_params(dst, grads != 0, false)) return fail(ION_ERR_DEVICE, h->net->error());
+    return ION_OK;
+}
+
+int ion_surrogate_write_params(ion_handle* h, const float* src, uint64_t n) {
+    int rc = need_net(h);
+    if (rc) return rc;
+    if (!src || n != h->net->param_count()) return fail(ION_ERR_INVALID, "parameter count mismatch");
+    if (h->net->copy_params(const_cast<float*>(src), false, true)) return fail(ION_ERR_DEVICE, h->net->error());
+    HIPCHK(hipStreamSynchronize(h->stream));
     return ION_OK;
 }
 

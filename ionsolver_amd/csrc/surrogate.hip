@@ -1,0 +1,415 @@
+// Learned E/B field surrogate: a linear 4-level 3D convolutional neural
+// operator on the level-of-detail (LOD) grid, with BF16 weights and
+// activations and FP32 accumulation on the MFMA units.
+//
+// Input: the top-level LOD sums (charge q and current J = q v per LOD block,
+// the data update_e_b_dynamic reads), output: E - E_stat and B - B_stat at
+// every cell. All layers are convolutions, so every layer is one GEMM over an
+// im2col matrix (bf16, K padded to 32): C[M][N] = A[M][K] . B[N][K]^T with
+// v_mfma_f32_16x16x32_bf16. The output head is a sub-pixel convolution: for
+// each LOD block it maps the 3^3 neighbourhood of level-0 features to the 6
+// field components of every cell of the block, and its epilogue writes the
+// fields (plus E_stat / B_stat) straight into E_dyn / B_dyn.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+#include "surrogate.hpp"
+
+namespace ion {
+namespace sur {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __bf16 f2bf(float x) { return (__bf16)x; }  // round to nearest even
+__device__ __forceinline__ float bf2f(__bf16 x) { return (float)x; }
+
+// ---------------------------------------------------------------------------
+// GEMM: C[M][N] = alpha * A[M][K] . B[N][K]^T (+ C if accumulate), bf16
+// operands with leading dimensions lda/ldb (elements), K % 32 == 0.
+// 64 x 64 tile per 256-thread block, 4 waves of 32 x 32 (2 x 2 MFMA tiles).
+// ---------------------------------------------------------------------------
+constexpr int BM = 64, BN = 64, BK = 32, LDS_PAD = 8;
+
+template <int EPI>
+__global__ void __launch_bounds__(256) k_gemm_nt(const __bf16* __restrict__ A, const __bf16* __restrict__ B, int M,
+                                                 int N, int K, int lda, int ldb, GemmOut out) {
+    __shared__ __bf16 As[BM][BK + LDS_PAD];
+    __shared__ __bf16 Bs[BN][BK + LDS_PAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+    const int lr = tid >> 2, lk = (tid & 3) * 8;  // loader: row, k offset (8 bf16 = 16 bytes)
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        bf16x8 va = (bf16x8){}, vb = (bf16x8){};
+        if (m0 + lr < M) va = *(const bf16x8*)(A + (size_t)(m0 + lr) * lda + k0 + lk);
+        if (n0 + lr < N) vb = *(const bf16x8*)(B + (size_t)(n0 + lr) * ldb + k0 + lk);
+        __syncthreads();
+        *(bf16x8*)&As[lr][lk] = va;
+        *(bf16x8*)&Bs[lr][lk] = vb;
+        __syncthreads();
+        const int fr = lane & 15, fk = (lane >> 4) * 8;
+        bf16x8 fa[2], fb[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) fa[i] = *(const bf16x8*)&As[wm + i * 16 + fr][fk];
+#pragma unroll
+        for (int j = 0; j < 2; j++) fb[j] = *(const bf16x8*)&Bs[wn + j * 16 + fr][fk];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    }
+    // lane holds C[(lane/16)*4 + r][lane%16] of each 16 x 16 tile
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+                const int n = n0 + wn + j * 16 + (lane & 15);
+                if (m >= M || n >= N) continue;
+                const float v = out.alpha * acc[i][j][r];
+                if (EPI == EPI_F32) {
+                    float* p = out.f32 + (size_t)m * out.ldc + n;
+                    *p = out.accumulate ? *p + v : v;
+                } else if (EPI == EPI_BF16) {
+                    out.bf16[(size_t)m * out.ldc + n] = f2bf(v);
+                } else {  // EPI_HEAD: m = LOD block, n = component * cells_per_block + offset
+                    const uint32_t c = (uint32_t)n / out.cpb, o = (uint32_t)n - c * out.cpb;
+                    const uint32_t ox = o % out.bx, oy = (o / out.bx) % out.by, oz = o / (out.bx * out.by);
+                    const uint32_t X = (uint32_t)m % out.nd, Y = ((uint32_t)m / out.nd) % out.nd,
+                                   Z = (uint32_t)m / (out.nd * out.nd);
+                    const uint64_t cell = (uint64_t)(X * out.bx + ox) +
+                                          ((uint64_t)(Y * out.by + oy) + (uint64_t)(Z * out.bz + oz) * out.ny) * out.nx;
+                    if (out.fine != nullptr) {  // training: raw normalised output
+                        out.fine[(size_t)c * out.ncell + cell] = acc[i][j][r];
+                    } else {
+                        if ((out.flags[cell] & 0x1F) == 0x01) continue;
+                        const float s = c < 3 ? out.scale_e : out.scale_b;
+                        const float* stat = c < 3 ? out.e_stat : out.b_stat;
+                        float* dyn = c < 3 ? out.e_dyn : out.b_dyn;
+                        const size_t idx = (size_t)(c % 3) * out.ncell + cell;
+                        dyn[idx] = stat[idx] + (*out.in_scale) * s * acc[i][j][r];
+                    }
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Network input: normalised (q, J) of the top-level LODs as bf16 hi + lo
+// parts (8 channels), zero for the LOD indices below `lo` that the direct
+// field update never reads (sim_kernels.cl:943).
+// ---------------------------------------------------------------------------
+__global__ void k_input_scale(const float4* __restrict__ qu, uint32_t n_top, float* __restrict__ scale) {
+    __shared__ float sh[256];
+    float m = 0.0f;
+    for (uint32_t i = threadIdx.x; i < n_top; i += blockDim.x) m = fmaxf(m, fabsf(qu[i].x));
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (uint32_t s = 128; s >= 1; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *scale = sh[0] > 0.0f ? sh[0] : 1.0f;
+}
+
+__global__ void k_input(const float4* __restrict__ qu, uint32_t n_top, uint32_t lo, const float* __restrict__ scale,
+                        __bf16* __restrict__ x) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n_top) return;
+    const float4 e = qu[v];
+    const float inv = 1.0f / *scale;
+    float in[4] = {e.x * inv, e.x * e.y * inv, e.x * e.z * inv, e.x * e.w * inv};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float val = v < lo ? 0.0f : in[c];
+        const __bf16 hi = f2bf(val);
+        x[(size_t)v * 8 + c] = hi;
+        x[(size_t)v * 8 + 4 + c] = f2bf(val - bf2f(hi));
+    }
+}
+
+// im2col of a 3^3 convolution over a g^3 grid (zero padding) of the channel
+// concatenation [a (ca channels), b (cb channels)]: row v, column
+// k = delta * (ca + cb) + ci, delta = (dx+1) + 3 (dy+1) + 9 (dz+1); columns
+// k >= 27 (ca + cb) are zero. transposed != 0 writes the [kpad][ldt] layout.
+__global__ void k_im2col(const __bf16* __restrict__ a, int ca, const __bf16* __restrict__ b, int cb, int g, int kpad,
+                         __bf16* __restrict__ out, int transposed, int ldt) {
+    const int cin = ca + cb;
+    const size_t total = (size_t)g * g * g * kpad;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int v = (int)(idx / kpad), k = (int)(idx % kpad);
+        __bf16 val = f2bf(0.0f);
+        if (k < 27 * cin) {
+            const int d = k / cin, ci = k % cin;
+            const int x = v % g + d % 3 - 1, y = (v / g) % g + (d / 3) % 3 - 1, z = v / (g * g) + d / 9 - 1;
+            if (x >= 0 && y >= 0 && z >= 0 && x < g && y < g && z < g) {
+                const int w = x + g * (y + g * z);
+                val = ci < ca ? a[(size_t)w * ca + ci] : b[(size_t)w * cb + (ci - ca)];
+            }
+        }
+        if (transposed) out[(size_t)k * ldt + v] = val;
+        else out[idx] = val;
+    }
+}
+
+// col2im (adjoint of im2col): d_in[w][ci] = sum over delta of dA[w - delta][delta*cin + ci],
+// written split into the two concatenated inputs (fp32 gradients).
+__global__ void k_col2im(const float* __restrict__ dA, int g, int kpad, int ca, int cb, float* __restrict__ da,
+                         float* __restrict__ db) {
+    const int cin = ca + cb;
+    const size_t total = (size_t)g * g * g * cin;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int w = (int)(idx / cin), ci = (int)(idx % cin);
+        const int wx = w % g, wy = (w / g) % g, wz = w / (g * g);
+        float s = 0.0f;
+        for (int d = 0; d < 27; d++) {
+            const int x = wx - (d % 3 - 1), y = wy - ((d / 3) % 3 - 1), z = wz - (d / 9 - 1);
+            if (x < 0 || y < 0 || z < 0 || x >= g || y >= g || z >= g) continue;
+            s += dA[(size_t)(x + g * (y + g * z)) * kpad + d * cin + ci];
+        }
+        if (ci < ca) {
+            if (da) da[(size_t)w * ca + ci] = s;
+        } else if (db) {
+            db[(size_t)w * cb + (ci - ca)] = s;
+        }
+    }
+}
+
+// 2x2x2 average pooling (g -> g/2) and nearest upsampling (g/2 -> g), bf16
+// activations; the backward versions work on fp32 gradients.
+__global__ void k_pool(const __bf16* __restrict__ in, int g, int c, __bf16* __restrict__ out) {
+    const int h = g / 2;
+    const size_t total = (size_t)h * h * h * c;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int v = (int)(idx / c), ch = (int)(idx % c);
+        const int x = 2 * (v % h), y = 2 * ((v / h) % h), z = 2 * (v / (h * h));
+        float s = 0.0f;
+        for (int d = 0; d < 8; d++)
+            s += bf2f(in[(size_t)((x + (d & 1)) + g * ((y + ((d >> 1) & 1)) + g * (z + (d >> 2)))) * c + ch]);
+        out[idx] = f2bf(0.125f * s);
+    }
+}
+__global__ void k_pool_bwd(const float* __restrict__ dout, int g, int c, float* __restrict__ din, int accumulate) {
+    const int h = g / 2;
+    const size_t total = (size_t)g * g * g * c;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int v = (int)(idx / c), ch = (int)(idx % c);
+        const int x = (v % g) / 2, y = ((v / g) % g) / 2, z = (v / (g * g)) / 2;
+        const float val = 0.125f * dout[(size_t)(x + h * (y + h * z)) * c + ch];
+        din[idx] = accumulate ? din[idx] + val : val;
+    }
+}
+__global__ void k_up(const __bf16* __restrict__ in, int h, int c, __bf16* __restrict__ out) {
+    const int g = 2 * h;
+    const size_t total = (size_t)g * g * g * c;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int v = (int)(idx / c), ch = (int)(idx % c);
+        const int x = (v % g) / 2, y = ((v / g) % g) / 2, z = (v / (g * g)) / 2;
+        out[idx] = in[(size_t)(x + h * (y + h * z)) * c + ch];
+    }
+}
+__global__ void k_up_bwd(const float* __restrict__ dout, int h, int c, float* __restrict__ din) {
+    const int g = 2 * h;
+    const size_t total = (size_t)h * h * h * c;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int v = (int)(idx / c), ch = (int)(idx % c);
+        const int x = 2 * (v % h), y = 2 * ((v / h) % h), z = 2 * (v / (h * h));
+        float s = 0.0f;
+        for (int d = 0; d < 8; d++)
+            s += dout[(size_t)((x + (d & 1)) + g * ((y + ((d >> 1) & 1)) + g * (z + (d >> 2)))) * c + ch];
+        din[idx] = s;
+    }
+}
+
+// fp32 [rows][cols] -> bf16 [rows][ld] (zero in padding columns), or transposed
+// to [ld][ldt] (ldt >= rows; columns rows..ldt-1 are left untouched).
+__global__ void k_to_bf16(const float* __restrict__ in, int rows, int cols, int ld, __bf16* __restrict__ out,
+                          int transposed, int ldt) {
+    const size_t total = (size_t)rows * ld;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(idx / ld), cc = (int)(idx % ld);
+        const __bf16 v = f2bf(cc < cols ? in[(size_t)r * cols + cc] : 0.0f);
+        if (transposed) {
+            out[(size_t)cc * ldt + r] = v;  // out is [ld][ldt], ldt >= rows
+        } else {
+            out[idx] = v;
+        }
+    }
+}
+
+// Loss of the head output y (network units, fp32 [6][N]) against the direct
+// field's raw source sums (E then B, [6][N]) in network units,
+// t = raw * gain / s, over fluid cells: per-block partial sums of (y - t)^2
+// and t^2 for the E and B groups.
+__global__ void __launch_bounds__(256) k_loss_sums(const float* __restrict__ y, const float* __restrict__ target,
+                                                   const uint8_t* __restrict__ flags, uint64_t ncell,
+                                                   const float* __restrict__ in_scale, float gain,
+                                                   double* __restrict__ partial) {
+    __shared__ double sh[4][256];
+    double s[4] = {0, 0, 0, 0};
+    const float tg = gain / *in_scale;
+    for (uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; n < ncell; n += (uint64_t)gridDim.x * blockDim.x) {
+        if ((flags[n] & 0x1F) == 0x01) continue;
+        for (int c = 0; c < 6; c++) {
+            const size_t i = (size_t)c * ncell + n;
+            const float t = target[i] * tg;
+            const float d = y[i] - t;
+            s[(c < 3 ? 0 : 2) + 0] += (double)d * d;
+            s[(c < 3 ? 0 : 2) + 1] += (double)t * t;
+        }
+    }
+    for (int k = 0; k < 4; k++) sh[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (uint32_t st = 128; st >= 1; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int k = 0; k < 4; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) partial[(size_t)blockIdx.x * 4 + k] = sh[k][0];
+}
+
+// dL/dy for L = sum_E (y-t)^2 / sum_E t^2 + sum_B (y-t)^2 / sum_B t^2, written as
+// bf16 in the two GEMM layouts dY[block][n] and dYt[n][block] (n = c*cpb + o);
+// the padding columns of both stay zero.
+__global__ void k_loss_grad(const float* __restrict__ y, const float* __restrict__ target,
+                            const uint8_t* __restrict__ flags, HeadGeom hg, const float* __restrict__ in_scale,
+                            float gain, const float* __restrict__ norms, __bf16* __restrict__ dY,
+                            __bf16* __restrict__ dYt) {
+    const uint64_t ncell = hg.ncell;
+    const float tg = gain / *in_scale;
+    const float ge = norms[0] > 0.0f ? 2.0f / norms[0] : 0.0f, gb = norms[1] > 0.0f ? 2.0f / norms[1] : 0.0f;
+    for (uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; n < ncell; n += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t z = (uint32_t)(n / ((uint64_t)hg.nx * hg.ny)), r = (uint32_t)(n % ((uint64_t)hg.nx * hg.ny));
+        const uint32_t yy = r / hg.nx, x = r % hg.nx;
+        const uint32_t blk = x / hg.bx + (yy / hg.by + (z / hg.bz) * hg.nd) * hg.nd;
+        const uint32_t o = (x % hg.bx) + ((yy % hg.by) + (z % hg.bz) * hg.by) * hg.bx;
+        const bool fluid = (flags[n] & 0x1F) != 0x01;
+        for (int c = 0; c < 6; c++) {
+            const size_t i = (size_t)c * ncell + n;
+            const float g = fluid ? (c < 3 ? ge : gb) * (y[i] - target[i] * tg) : 0.0f;
+            const uint32_t col = c * hg.cpb + o;
+            dY[(size_t)blk * hg.ld_dy + col] = f2bf(g);
+            dYt[(size_t)col * hg.ld_dyt + blk] = f2bf(g);
+        }
+    }
+}
+
+// Adam on fp32 master weights; refreshes the bf16 copy ([rows][ld] with the
+// master's [rows][cols] data and zero padding).
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       size_t n, float lr, float b1, float b2, float eps, float bc1, float bc2) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    }
+}
+
+// Adam for the head matrix [rows][kpad]: columns of level-0 features
+// (k % hcin < c0) step with lr_feat, columns of the direct source channels
+// with lr_src.
+__global__ void k_adam_head(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, size_t n, int kpad, int hcin, int c0, float lr_feat, float lr_src,
+                            float b1, float b2, float eps, float bc1, float bc2) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i % (size_t)kpad);
+        const float lr = (k % hcin) < c0 ? lr_feat : lr_src;
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    }
+}
+
+// ---------------------------------------------------------------------------
+static inline dim3 grid_for(size_t n) {
+    size_t b = (n + 255) / 256;
+    if (b > 65535) b = 65535;
+    if (b == 0) b = 1;
+    return dim3((uint32_t)b);
+}
+
+hipError_t gemm_nt(int epi, const __bf16* A, const __bf16* B, int M, int N, int K, int lda, int ldb, const GemmOut& o,
+                   hipStream_t s) {
+    if (K % BK != 0 || M <= 0 || N <= 0) return hipErrorInvalidValue;
+    const dim3 g((N + BN - 1) / BN, (M + BM - 1) / BM);
+    if (epi == EPI_F32) hipLaunchKernelGGL(k_gemm_nt<EPI_F32>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
+    else if (epi == EPI_BF16) hipLaunchKernelGGL(k_gemm_nt<EPI_BF16>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
+    else hipLaunchKernelGGL(k_gemm_nt<EPI_HEAD>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
+    return hipGetLastError();
+}
+hipError_t input(const float4* qu, uint32_t n_top, uint32_t lo, float* scale, __bf16* x, hipStream_t s) {
+    hipLaunchKernelGGL(k_input_scale, dim3(1), dim3(256), 0, s, qu, n_top, scale);
+    hipLaunchKernelGGL(k_input, dim3((n_top + 255) / 256), dim3(256), 0, s, qu, n_top, lo, scale, x);
+    return hipGetLastError();
+}
+hipError_t im2col(const __bf16* a, int ca, const __bf16* b, int cb, int g, int kpad, __bf16* out, int transposed,
+                  int ldt, hipStream_t s) {
+    hipLaunchKernelGGL(k_im2col, grid_for((size_t)g * g * g * kpad), dim3(256), 0, s, a, ca, b, cb, g, kpad, out,
+                       transposed, ldt);
+    return hipGetLastError();
+}
+hipError_t col2im(const float* dA, int g, int kpad, int ca, int cb, float* da, float* db, hipStream_t s) {
+    hipLaunchKernelGGL(k_col2im, grid_for((size_t)g * g * g * (ca + cb)), dim3(256), 0, s, dA, g, kpad, ca, cb, da, db);
+    return hipGetLastError();
+}
+hipError_t pool(const __bf16* in, int g, int c, __bf16* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_pool, grid_for((size_t)(g / 2) * (g / 2) * (g / 2) * c), dim3(256), 0, s, in, g, c, out);
+    return hipGetLastError();
+}
+hipError_t pool_bwd(const float* dout, int g, int c, float* din, int accumulate, hipStream_t s) {
+    hipLaunchKernelGGL(k_pool_bwd, grid_for((size_t)g * g * g * c), dim3(256), 0, s, dout, g, c, din, accumulate);
+    return hipGetLastError();
+}
+hipError_t up(const __bf16* in, int h, int c, __bf16* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_up, grid_for((size_t)8 * h * h * h * c), dim3(256), 0, s, in, h, c, out);
+    return hipGetLastError();
+}
+hipError_t up_bwd(const float* dout, int h, int c, float* din, hipStream_t s) {
+    hipLaunchKernelGGL(k_up_bwd, grid_for((size_t)h * h * h * c), dim3(256), 0, s, dout, h, c, din);
+    return hipGetLastError();
+}
+hipError_t to_bf16(const float* in, int rows, int cols, int ld, __bf16* out, int transposed, int ldt, hipStream_t s) {
+    hipLaunchKernelGGL(k_to_bf16, grid_for((size_t)rows * ld), dim3(256), 0, s, in, rows, cols, ld, out, transposed, ldt);
+    return hipGetLastError();
+}
+hipError_t loss_sums(const float* y, const float* target, const uint8_t* flags, uint64_t ncell, const float* in_scale,
+                     float gain, double* partial, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_loss_sums, dim3(nblocks), dim3(256), 0, s, y, target, flags, ncell, in_scale, gain, partial);
+    return hipGetLastError();
+}
+hipError_t loss_grad(const float* y, const float* target, const uint8_t* flags, const HeadGeom& hg,
+                     const float* in_scale, float gain, const float* norms, __bf16* dY, __bf16* dYt, hipStream_t s) {
+    hipLaunchKernelGGL(k_loss_grad, grid_for(hg.ncell), dim3(256), 0, s, y, target, flags, hg, in_scale, gain, norms,
+                       dY, dYt);
+    return hipGetLastError();
+}
+hipError_t adam_head(float* w, const float* g, float* m, float* v, size_t n, int kpad, int hcin, int c0, float lr_feat,
+                     float lr_src, float b1, float b2, float eps, float bc1, float bc2, hipStream_t s) {
+    hipLaunchKernelGGL(k_adam_head, grid_for(n), dim3(256), 0, s, w, g, m, v, n, kpad, hcin, c0, lr_feat, lr_src, b1,
+                       b2, eps, bc1, bc2);
+    return hipGetLastError();
+}
+hipError_t adam(float* w, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2, float eps,
+                float bc1, float bc2, hipStream_t s) {
+    hipLaunchKernelGGL(k_adam, grid_for(n), dim3(256), 0, s, w, g, m, v, n, lr, b1, b2, eps, bc1, bc2);
+    return hipGetLastError();
+}
+
+}  // namespace sur
+}  // namespace ion

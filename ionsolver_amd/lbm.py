@@ -71,6 +71,7 @@ class Field(enum.IntEnum):
     Q = 10
     QU_LOD = 11
     EI = 12
+    FIELD_SUMS = 13   # raw direct source sums (E then B, 6 x N), once a surrogate exists
 
 
 class FieldMode(enum.IntEnum):
@@ -114,6 +115,22 @@ class _Timing(ctypes.Structure):
                 ("ms_other", ctypes.c_double), ("n_steps", ctypes.c_uint64)]
 
 
+class _SurConfig(ctypes.Structure):
+    _fields_ = [("channels", ctypes.c_int32), ("seed", ctypes.c_uint64), ("lr", ctypes.c_float),
+                ("lr_head", ctypes.c_float)]
+
+
+class _SurStats(ctypes.Structure):
+    _fields_ = [("train_steps", ctypes.c_uint64), ("direct_steps", ctypes.c_uint64),
+                ("surrogate_steps", ctypes.c_uint64), ("fallbacks", ctypes.c_uint64),
+                ("last_residual_e", ctypes.c_double), ("last_residual_b", ctypes.c_double)]
+
+
+class FieldSolver(enum.IntEnum):
+    DIRECT = 0
+    SURROGATE = 1
+
+
 # Every entry point of include/ionsolver.h: name -> (restype, argtypes)
 _VP, _I, _U64, _F, _D, _SZ = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_float, ctypes.c_double, ctypes.c_size_t
 ABI = {
@@ -142,6 +159,18 @@ ABI = {
     "ion_get_timing": (_I, [_VP, ctypes.POINTER(_Timing)]),
     "ion_device_codec": (_I, [_I, _I, _VP, _VP, _U64]),
     "ion_last_error": (ctypes.c_char_p, []),
+    "ion_surrogate_config_default": (None, [ctypes.POINTER(_SurConfig)]),
+    "ion_surrogate_create": (_I, [_VP, ctypes.POINTER(_SurConfig)]),
+    "ion_compute_fields": (_I, [_VP]),
+    "ion_surrogate_infer": (_I, [_VP]),
+    "ion_surrogate_train_step": (_I, [_VP, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "ion_surrogate_residual": (_I, [_VP, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "ion_set_field_solver": (_I, [_VP, _I, ctypes.c_uint32, _D]),
+    "ion_surrogate_set_learning_rate": (_I, [_VP, ctypes.c_float, ctypes.c_float]),
+    "ion_surrogate_get_stats": (_I, [_VP, ctypes.POINTER(_SurStats)]),
+    "ion_surrogate_param_count": (_I, [_VP, ctypes.POINTER(_U64)]),
+    "ion_surrogate_read_params": (_I, [_VP, _I, _VP, _U64]),
+    "ion_surrogate_write_params": (_I, [_VP, _VP, _U64]),
 }
 
 _lib = None
@@ -396,6 +425,54 @@ class Lbm:
         t = _Timing()
         _check(self._lib.ion_get_timing(self._h, ctypes.byref(t)))
         return {f: getattr(t, f) for f, _ in t._fields_}
+
+
+    # learned field surrogate
+    def surrogate_create(self, channels=16, seed=1, lr=1e-3, lr_head=1e-4):
+        c = _SurConfig()
+        self._lib.ion_surrogate_config_default(ctypes.byref(c))
+        c.channels, c.seed, c.lr, c.lr_head = channels, seed, lr, lr_head
+        _check(self._lib.ion_surrogate_create(self._h, ctypes.byref(c)))
+
+    def compute_fields(self):
+        """Direct field update from the current LOD sums (enqueue_update_e_b_dyn)."""
+        _check(self._lib.ion_compute_fields(self._h))
+
+    def surrogate_infer(self):
+        _check(self._lib.ion_surrogate_infer(self._h))
+
+    def surrogate_train_step(self, update=True):
+        e, b = ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.ion_surrogate_train_step(self._h, int(bool(update)), ctypes.byref(e), ctypes.byref(b)))
+        return e.value, b.value
+
+    def surrogate_residual(self):
+        e, b = ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.ion_surrogate_residual(self._h, ctypes.byref(e), ctypes.byref(b)))
+        return e.value, b.value
+
+    def surrogate_set_learning_rate(self, lr, lr_head):
+        _check(self._lib.ion_surrogate_set_learning_rate(self._h, float(lr), float(lr_head)))
+
+    def set_field_solver(self, solver, judge_every=50, tolerance=0.01):
+        _check(self._lib.ion_set_field_solver(self._h, int(solver), int(judge_every), float(tolerance)))
+
+    def surrogate_stats(self):
+        st = _SurStats()
+        _check(self._lib.ion_surrogate_get_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    def surrogate_params(self, grads=False):
+        n = ctypes.c_uint64()
+        _check(self._lib.ion_surrogate_param_count(self._h, ctypes.byref(n)))
+        out = np.empty(n.value, np.float32)
+        _check(self._lib.ion_surrogate_read_params(self._h, int(bool(grads)), out.ctypes.data_as(ctypes.c_void_p),
+                                                   n.value))
+        return out
+
+    def set_surrogate_params(self, params):
+        arr = np.ascontiguousarray(params, np.float32)
+        _check(self._lib.ion_surrogate_write_params(self._h, arr.ctypes.data_as(ctypes.c_void_p), arr.size))
 
 
 def device_codec(float_type, direction, data):

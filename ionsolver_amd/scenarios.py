@@ -103,3 +103,35 @@ def apply(lbm, arrays):
                  F=Field.F)
     for k, v in arrays.items():
         lbm.write_field(names[k], v)
+
+
+def random_lod(shape, D, rng, n_blobs=8, sigma=(3.0, 16.0), amp=0.05, vel=0.05, noise=0.02):
+    """Top-level LOD sums (q, mean v) of a random charged flow, as written by
+    stream_collide: 'n_blobs' Gaussian charge blobs (random centre, width,
+    sign and amplitude) integrated over each LOD block (separable, exact cell
+    sums), plus white block noise, and a smooth random velocity field.
+    Returns the [n_top][4] float32 array for the LOD buffer's first entries."""
+    nx, ny, nz = shape
+    nd = 1 << D
+    b = (nx // nd, ny // nd, nz // nd)
+    q = np.zeros((nd, nd, nd))
+    for _ in range(n_blobs):
+        s = rng.uniform(*sigma)
+        a = (1.0 if rng.uniform() < 0.5 else -1.0) * amp * rng.uniform(0.5, 1.0)
+        prof = []
+        for L, bb in zip((nx, ny, nz), b):
+            c = rng.uniform(0, L)
+            g = np.exp(-(np.arange(L) - c) ** 2 / (2 * s * s))
+            prof.append(g.reshape(nd, bb).sum(1))
+        q += a * prof[2][:, None, None] * prof[1][None, :, None] * prof[0][None, None, :]
+    q += noise * amp * np.prod(b) * rng.standard_normal(q.shape) / np.sqrt(np.prod(b))
+    Z, Y, X = np.meshgrid(*(np.arange(nd) + 0.5,) * 3, indexing="ij")
+    v = []
+    for _ in range(3):
+        f = rng.uniform(-1, 1) * 0.5
+        for _ in range(3):
+            k = rng.integers(1, 4, 3) * 2 * np.pi / nd
+            f = f + rng.uniform(-1, 1) * np.sin(k[0] * X + k[1] * Y + k[2] * Z + rng.uniform(0, 2 * np.pi))
+        v.append(vel * f / 2)
+    out = np.stack([q.ravel(), v[0].ravel(), v[1].ravel(), v[2].ravel()], 1)
+    return out.astype(np.float32)

@@ -1,0 +1,206 @@
+"""Float64 reference of the learned field surrogate (test infrastructure only:
+the checker of the GPU network's forward output and gradients).
+
+Restates the network of ionsolver_amd/csrc/surrogate_host.cpp in numpy:
+input (q, J = q v) of the top-level LOD blocks / s as bf16 hi + lo parts,
+a linear 4-level 3D conv U-Net over the LOD grid (3^3 convolutions, 2^3
+average pooling, nearest upsampling, skip concatenation), and the sub-pixel
+output head that maps the 3^3 neighbourhood of [d0, x] to the 6 field
+components of every cell of a LOD block. Loss: relative squared error of E
+and of B over fluid cells. The backward pass is written out by hand; the
+tests check it against central finite differences.
+
+Parameter layout (flat): layers e0, e1, e2, e3, d2, d1, d0, head, each a
+[cout][kpad] matrix with column k = delta * cin + ci, delta = (dx+1) +
+3 (dy+1) + 9 (dz+1), kpad = 27 cin rounded up to a multiple of 32.
+"""
+import numpy as np
+
+LAYERS = ["e0", "e1", "e2", "e3", "d2", "d1", "d0"]
+
+
+def pad32(v):
+    return (v + 31) // 32 * 32
+
+
+def bf16(x):
+    """Round float64/float32 values to bfloat16 (nearest even), as float64."""
+    f = np.asarray(x, np.float32)
+    b = f.view(np.uint32).astype(np.uint64)
+    rounded = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    out = rounded.astype(np.uint32).view(np.float32).astype(np.float64)
+    return np.where(np.isnan(f), np.nan, out)
+
+
+class Geometry:
+    def __init__(self, nx, ny, nz, D, c0):
+        self.nx, self.ny, self.nz, self.D, self.c0 = nx, ny, nz, D, c0
+        self.nd = 1 << D
+        self.bx, self.by, self.bz = nx // self.nd, ny // self.nd, nz // self.nd
+        self.cpb = self.bx * self.by * self.bz
+        self.nout = 6 * self.cpb
+        self.gain = float(np.float32(self.cpb ** (2.0 / 3.0)))
+        self.nblk = self.nd ** 3
+        self.lo = sum(8 ** k for k in range(D + 1)) - self.nblk
+        C = [c0, 2 * c0, 3 * c0, 4 * c0]
+        g0 = self.nd
+        # (cin_a, cin_b, cout, grid)
+        self.layer_dims = {
+            "e0": (8, 0, C[0], g0), "e1": (C[0], 0, C[1], g0 // 2), "e2": (C[1], 0, C[2], g0 // 4),
+            "e3": (C[2], 0, C[3], g0 // 8), "d2": (C[3], C[2], C[2], g0 // 4), "d1": (C[2], C[1], C[1], g0 // 2),
+            "d0": (C[1], C[0], C[0], g0),
+        }
+        self.hcin = c0 + 8
+        self.hk = pad32(27 * self.hcin)
+        self.shapes = [(d[2], pad32(27 * (d[0] + d[1]))) for d in (self.layer_dims[n] for n in LAYERS)]
+        self.shapes.append((self.nout, self.hk))
+
+    def split(self, flat):
+        out, off = [], 0
+        for r, c in self.shapes:
+            out.append(flat[off:off + r * c].reshape(r, c))
+            off += r * c
+        assert off == flat.size
+        return out
+
+    def n_params(self):
+        return sum(r * c for r, c in self.shapes)
+
+
+def conv3(x, W, cin):
+    """x: (g, g, g, cin) [z, y, x, c]; W: (cout, kpad) -> (g, g, g, cout)."""
+    g = x.shape[0]
+    p = np.pad(x, ((1, 1), (1, 1), (1, 1), (0, 0)))
+    out = np.zeros((g, g, g, W.shape[0]))
+    for d in range(27):
+        dx, dy, dz = d % 3 - 1, (d // 3) % 3 - 1, d // 9 - 1
+        sl = p[1 + dz:1 + dz + g, 1 + dy:1 + dy + g, 1 + dx:1 + dx + g, :]
+        out += sl @ W[:, d * cin:(d + 1) * cin].T
+    return out
+
+
+def conv3_bwd(x, W, cin, dout):
+    """Gradients of conv3 w.r.t. W (full kpad width) and x."""
+    g = x.shape[0]
+    p = np.pad(x, ((1, 1), (1, 1), (1, 1), (0, 0)))
+    dW = np.zeros_like(W)
+    dp = np.zeros_like(p)
+    for d in range(27):
+        dx, dy, dz = d % 3 - 1, (d // 3) % 3 - 1, d // 9 - 1
+        sl = (slice(1 + dz, 1 + dz + g), slice(1 + dy, 1 + dy + g), slice(1 + dx, 1 + dx + g), slice(None))
+        dW[:, d * cin:(d + 1) * cin] = np.einsum("zyxo,zyxc->oc", dout, p[sl])
+        dp[sl] += dout @ W[:, d * cin:(d + 1) * cin]
+    return dW, dp[1:-1, 1:-1, 1:-1, :]
+
+
+def pool(x):
+    g = x.shape[0]
+    return x.reshape(g // 2, 2, g // 2, 2, g // 2, 2, -1).mean(axis=(1, 3, 5))
+
+
+def pool_bwd(d):
+    return np.repeat(np.repeat(np.repeat(d, 2, 0), 2, 1), 2, 2) / 8.0
+
+
+def up(x):
+    return np.repeat(np.repeat(np.repeat(x, 2, 0), 2, 1), 2, 2)
+
+
+def up_bwd(d):
+    g = d.shape[0]
+    return d.reshape(g // 2, 2, g // 2, 2, g // 2, 2, -1).sum(axis=(1, 3, 5))
+
+
+def net_input(geo, qu):
+    """(nblk, 4) LOD sums -> (g, g, g, 8) input, scale s."""
+    q = qu[:geo.nblk].astype(np.float64)
+    s = np.max(np.abs(np.float32(q[:, 0])))
+    s = float(s) if s > 0 else 1.0
+    v = np.stack([q[:, 0], q[:, 0] * q[:, 1], q[:, 0] * q[:, 2], q[:, 0] * q[:, 3]], 1)
+    v = np.float32(np.float32(v) * np.float32(1.0 / np.float32(s))).astype(np.float64)
+    v[:geo.lo] = 0.0
+    hi = bf16(v)
+    lo = bf16(v - hi)
+    x = np.concatenate([hi, lo], 1)
+    g = geo.nd
+    return x.reshape(g, g, g, 8), s
+
+
+def forward(geo, params, qu):
+    Ws = geo.split(params)
+    W = dict(zip(LAYERS, Ws[:7]))
+    Wh = Ws[7]
+    x, s = net_input(geo, qu)
+    a = {}
+    a["x"] = x
+    a["e0"] = conv3(x, W["e0"], 8)
+    a["p0"] = pool(a["e0"])
+    a["e1"] = conv3(a["p0"], W["e1"], geo.layer_dims["e1"][0])
+    a["p1"] = pool(a["e1"])
+    a["e2"] = conv3(a["p1"], W["e2"], geo.layer_dims["e2"][0])
+    a["p2"] = pool(a["e2"])
+    a["e3"] = conv3(a["p2"], W["e3"], geo.layer_dims["e3"][0])
+    a["u2"] = np.concatenate([up(a["e3"]), a["e2"]], 3)
+    a["d2"] = conv3(a["u2"], W["d2"], a["u2"].shape[3])
+    a["u1"] = np.concatenate([up(a["d2"]), a["e1"]], 3)
+    a["d1"] = conv3(a["u1"], W["d1"], a["u1"].shape[3])
+    a["u0"] = np.concatenate([up(a["d1"]), a["e0"]], 3)
+    a["d0"] = conv3(a["u0"], W["d0"], a["u0"].shape[3])
+    a["h"] = np.concatenate([a["d0"], x], 3)
+    Y = conv3(a["h"], Wh, geo.hcin)  # (g, g, g, nout) per block
+    return Y, s, a
+
+
+def block_to_cells(geo, Y):
+    """(g, g, g, 6*cpb) block outputs -> (6, ncell) fields."""
+    g = geo.nd
+    Y = Y.reshape(g, g, g, 6, geo.bz, geo.by, geo.bx)           # Z, Y, X, c, oz, oy, ox
+    Y = Y.transpose(3, 0, 4, 1, 5, 2, 6)                         # c, Z, oz, Y, oy, X, ox
+    return Y.reshape(6, geo.nx * geo.ny * geo.nz)
+
+
+def cells_to_block(geo, F):
+    g = geo.nd
+    F = F.reshape(6, g, geo.bz, g, geo.by, g, geo.bx).transpose(1, 3, 5, 0, 2, 4, 6)
+    return F.reshape(g, g, g, 6 * geo.cpb)
+
+
+def targets(geo, s, raw):
+    """Network-unit targets from the direct kernel's raw source sums
+    (E then B, 6 x N): raw * gain / s, gain = (cells per block)^(2/3)."""
+    return raw.reshape(6, -1).astype(np.float64) * (geo.gain / s)
+
+
+def loss_and_grad(geo, params, qu, T, fluid):
+    """Loss sum_E (y-t)^2/sum_E t^2 + sum_B (y-t)^2/sum_B t^2 over fluid cells,
+    its gradient w.r.t. the flat parameters, and (rel_e, rel_b)."""
+    Y, s, a = forward(geo, params, qu)
+    y = block_to_cells(geo, Y)
+    m = fluid[None, :]
+    d = (y - T) * m
+    nE, nB = np.sum((T[:3] * m) ** 2), np.sum((T[3:] * m) ** 2)
+    L = np.sum(d[:3] ** 2) / nE + np.sum(d[3:] ** 2) / nB
+    dy = np.concatenate([2 * d[:3] / nE, 2 * d[3:] / nB], 0)
+    dY = cells_to_block(geo, dy)
+    Ws = geo.split(params)
+    W = dict(zip(LAYERS, Ws[:7]))
+    grads = {}
+    grads["head"], dh = conv3_bwd(a["h"], Ws[7], geo.hcin, dY)
+    c0, C1, C2 = geo.c0, 2 * geo.c0, 3 * geo.c0
+    dd0 = dh[..., :c0]
+    grads["d0"], du0 = conv3_bwd(a["u0"], W["d0"], a["u0"].shape[3], dd0)
+    dd1, de0 = up_bwd(du0[..., :C1]), du0[..., C1:]
+    grads["d1"], du1 = conv3_bwd(a["u1"], W["d1"], a["u1"].shape[3], dd1)
+    dd2, de1 = up_bwd(du1[..., :C2]), du1[..., C2:]
+    grads["d2"], du2 = conv3_bwd(a["u2"], W["d2"], a["u2"].shape[3], dd2)
+    de3, de2 = up_bwd(du2[..., :4 * c0]), du2[..., 4 * c0:]
+    grads["e3"], dp2 = conv3_bwd(a["p2"], W["e3"], C2, de3)
+    de2 = de2 + pool_bwd(dp2)
+    grads["e2"], dp1 = conv3_bwd(a["p1"], W["e2"], C1, de2)
+    de1 = de1 + pool_bwd(dp1)
+    grads["e1"], dp0 = conv3_bwd(a["p0"], W["e1"], c0, de1)
+    de0 = de0 + pool_bwd(dp0)
+    grads["e0"], _ = conv3_bwd(a["x"], W["e0"], 8, de0)
+    flat = np.concatenate([grads[n].ravel() for n in LAYERS + ["head"]])
+    rel = (np.sqrt(np.sum(d[:3] ** 2) / nE), np.sqrt(np.sum(d[3:] ** 2) / nB))
+    return L, flat, rel, y

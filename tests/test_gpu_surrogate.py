@@ -1,0 +1,159 @@
+"""GPU checks of the learned E/B field surrogate (BF16 MFMA network).
+
+The surrogate is an approximation, so its bar is a tolerance, not bit
+equality: the GPU forward output, loss and gradients agree with the float64
+reference network (oracle/surrogate_ref.py) within the rounding of BF16
+operands (2^-9 relative per operand, FP32 accumulation); the initial head
+reproduces the direct near field of the 26 neighbouring LOD blocks; training
+lowers the residual against the direct field; and the judge falls back to
+the direct solver when the surrogate misses its tolerance.
+"""
+import numpy as np
+import pytest
+
+from harness import make_config, rel_l2
+from oracle import surrogate_ref as S
+
+import ionsolver_amd as I
+from ionsolver_amd import Field, FieldSolver
+from ionsolver_amd.scenarios import random_lod
+from ionsolver_amd.surrogate import evaluate, lod_buffer, train
+
+pytestmark = pytest.mark.gpu
+
+
+def make_lbm(n=16, D=3, solid_frac=0.0, seed=0, channels=8, **kw):
+    cfg = make_config(nx=n, ny=n, nz=n, mhd=True, lod_depth=D, graphics=False)
+    lbm = I.Lbm(cfg)
+    if solid_frac > 0:
+        rng = np.random.default_rng(seed)
+        flags = np.where(rng.uniform(size=n ** 3) < solid_frac, 1, 0).astype(np.uint8)
+        lbm.write_field(Field.FLAGS, flags)
+    rng = np.random.default_rng(seed + 1)
+    lbm.write_field(Field.E_STAT, (0.01 * rng.standard_normal(3 * n ** 3)).astype(np.float32))
+    lbm.write_field(Field.B_STAT, (0.01 * rng.standard_normal(3 * n ** 3)).astype(np.float32))
+    lbm.surrogate_create(channels=channels, **kw)
+    return lbm
+
+
+def test_initial_head_is_the_direct_near_field():
+    """All charge in one LOD block: the cells of its 26 neighbour blocks see
+    only near-field sources, which the initial head computes exactly (up to
+    the bf16 rounding of the head weights)."""
+    lbm = make_lbm(16, 3)
+    geo = S.Geometry(16, 16, 16, 3, 8)
+    # static fields zero: B_dyn - B_stat (~KMU = 1e-19) would vanish in the f32 sum
+    lbm.write_field(Field.E_STAT, np.zeros(3 * 16 ** 3, np.float32))
+    lbm.write_field(Field.B_STAT, np.zeros(3 * 16 ** 3, np.float32))
+    # keep only the head's direct-source columns (drop the U-Net's contribution)
+    p = lbm.surrogate_params()
+    head = p[-geo.nout * geo.hk:].reshape(geo.nout, geo.hk)
+    for d in range(27):
+        head[:, d * geo.hcin:d * geo.hcin + geo.c0] = 0.0
+    lbm.set_surrogate_params(p)
+    top = np.zeros((geo.nblk, 4), np.float32)
+    X, Y, Z = 3, 4, 5
+    top[X + 8 * (Y + 8 * Z)] = (0.7, 0.01, -0.02, 0.03)
+    lbm.write_field(Field.QU_LOD, lod_buffer(lbm, top))
+    lbm.compute_fields()
+    e_dir, b_dir = lbm.read_field(Field.E_DYN), lbm.read_field(Field.B_DYN)
+    lbm.surrogate_infer()
+    e_sur, b_sur = lbm.read_field(Field.E_DYN), lbm.read_field(Field.B_DYN)
+    e_st, b_st = lbm.read_field(Field.E_STAT), lbm.read_field(Field.B_STAT)
+    n = np.arange(16 ** 3)
+    x, y, z = n % 16 // 2, n // 16 % 16 // 2, n // 256 // 2
+    nb = (np.abs(x - X) <= 1) & (np.abs(y - Y) <= 1) & (np.abs(z - Z) <= 1) & ~((x == X) & (y == Y) & (z == Z))
+    m3 = np.tile(nb, 3)
+    assert np.abs(e_dir - e_st)[m3].max() > 0
+    assert rel_l2((e_sur - e_st)[m3], (e_dir - e_st)[m3]) < 1e-2
+    assert rel_l2((b_sur - b_st)[m3], (b_dir - b_st)[m3]) < 1e-2
+
+
+@pytest.mark.parametrize("solid_frac", [0.0, 0.2])
+def test_forward_loss_and_gradients_vs_float64_reference(solid_frac):
+    lbm = make_lbm(16, 3, solid_frac=solid_frac, seed=3)
+    geo = S.Geometry(16, 16, 16, 3, 8)
+    rng = np.random.default_rng(4)
+    params = np.concatenate([rng.standard_normal(r * c) / np.sqrt(27 * 8) for r, c in geo.shapes]).astype(np.float32)
+    lbm.set_surrogate_params(params)
+    np.testing.assert_array_equal(lbm.surrogate_params(), params)
+    top = random_lod((16, 16, 16), 3, rng)
+    lbm.write_field(Field.QU_LOD, lod_buffer(lbm, top))
+    lbm.compute_fields()
+    e_dyn = lbm.read_field(Field.E_DYN)
+    e_st = lbm.read_field(Field.E_STAT)
+    raw = lbm.read_field(Field.FIELD_SUMS)
+    rel_e, rel_b = lbm.surrogate_train_step(update=False)
+    grads = lbm.surrogate_params(grads=True)
+
+    k = lbm.config.kernel_constants()
+    fluid = (lbm.read_field(Field.FLAGS) & 0x1F) != 1
+    s = float(np.max(np.abs(top[:, 0])))
+    T = S.targets(geo, s, raw)
+    assert np.sum(T[3:] ** 2) > 0
+    L, g_ref, rel_ref, y_ref = S.loss_and_grad(geo, params.astype(np.float64), top, T, fluid.astype(np.float64))
+    assert rel_e == pytest.approx(rel_ref[0], rel=2e-2)
+    assert rel_b == pytest.approx(rel_ref[1], rel=2e-2)
+    offs = np.cumsum([0] + [r * c for r, c in geo.shapes])
+    for li, name in enumerate(S.LAYERS + ["head"]):
+        err = rel_l2(grads[offs[li]:offs[li + 1]], g_ref[offs[li]:offs[li + 1]])
+        assert err < 3e-2, (name, err)
+    # the parameters were not touched by a step without update
+    np.testing.assert_array_equal(lbm.surrogate_params(), params)
+
+    # inference writes E_stat + KE s y / gain (fluid cells only)
+    lbm.surrogate_infer()
+    e_sur = lbm.read_field(Field.E_DYN)
+    ref = e_st.astype(np.float64) + k["ke"] * s / geo.gain * y_ref[:3].ravel()
+    m3 = np.tile(fluid, 3)
+    assert rel_l2((e_sur - e_st)[m3], (ref - e_st)[m3]) < 2e-2
+    np.testing.assert_array_equal(e_sur[~m3], e_dyn[~m3])
+
+
+def test_training_lowers_the_residual():
+    lbm = make_lbm(32, 3, channels=8, lr=2e-3, lr_head=2e-4)
+    before = evaluate(lbm, 4)
+    hist = train(lbm, 150, seed=1)
+    after = evaluate(lbm, 4)
+    assert np.all(np.isfinite(hist))
+    assert after.mean() < 0.7 * before.mean(), (before.mean(0), after.mean(0))
+
+
+def test_judge_falls_back_to_the_direct_field():
+    """With an unreachable tolerance every field step runs the direct update
+    (bit-identical to a direct-solver run); with a loose one the surrogate
+    serves the steps between judgements."""
+    out = {}
+    for mode in ("direct", "strict", "loose"):
+        cfg = make_config(nx=16, ny=16, nz=16, mhd=True, lod_depth=3, graphics=False)
+        lbm = I.Lbm(cfg)
+        rng = np.random.default_rng(5)
+        n = 16 ** 3
+        lbm.write_field(Field.Q, (0.05 * rng.standard_normal(n)).astype(np.float32))
+        lbm.write_field(Field.B_STAT, np.full(3 * n, 0.01, np.float32))
+        if mode != "direct":
+            lbm.surrogate_create(channels=8)
+            lbm.set_field_solver(FieldSolver.SURROGATE, judge_every=3, tolerance=1e-9 if mode == "strict" else 1e9)
+        lbm.initialize()
+        lbm.run(7)
+        out[mode] = (lbm.read_field(Field.E_DYN), lbm.read_field(Field.B_DYN), lbm.surrogate_stats())
+    # the raw-sum + combine path of a surrogate run is bit-identical to the plain direct kernel
+    np.testing.assert_array_equal(out["strict"][0], out["direct"][0])
+    np.testing.assert_array_equal(out["strict"][1], out["direct"][1])
+    st = out["strict"][2]
+    assert st["surrogate_steps"] == 0 and st["fallbacks"] >= 1 and st["direct_steps"] >= 7
+    lo = out["loose"][2]
+    assert lo["surrogate_steps"] > 0 and lo["fallbacks"] == 0
+    assert lo["surrogate_steps"] + lo["direct_steps"] >= 7
+
+
+def test_surrogate_rejects_unsupported_configs():
+    cfg = make_config(nx=16, ny=16, nz=16, mhd=True, lod_depth=2, graphics=False)
+    lbm = I.Lbm(cfg)
+    with pytest.raises(I.IonError):
+        lbm.surrogate_create(channels=8)
+    lbm = I.Lbm(make_config(nx=16, ny=16, nz=16, mhd=True, lod_depth=3, graphics=False))
+    with pytest.raises(I.IonError):
+        lbm.surrogate_create(channels=12)
+    with pytest.raises(I.IonError):
+        lbm.set_field_solver(FieldSolver.SURROGATE)
