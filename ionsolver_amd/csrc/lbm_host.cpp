@@ -218,13 +218,18 @@ ion::LbmArgs lbm_args(ion_handle* h) {
 }
 
 // update_e_b_dynamic for the current buffers (sim_kernels.cl:897-993).
-int field_update(ion_handle* h) {
+// Compacted LOD sources (q != 0) of the range update_e_b_dynamic reads.
+int prep_sources(ion_handle* h) {
     const uint64_t n_top = (uint64_t)h->nd * h->nd * h->nd;
     const uint32_t hi = (uint32_t)h->n_lod, lo = (uint32_t)(h->n_lod - n_top);
-    int rc = timed(h, 2, [&] {
+    return timed(h, 2, [&] {
         return ion::launch_prep_sources((const float4*)h->qu_lod, lo, hi, h->cfg.n_x, h->cfg.n_y, h->cfg.n_z, h->D,
                                         h->src_pq, h->src_vd, h->src_count, h->stream);
     });
+}
+
+int field_update(ion_handle* h) {
+    int rc = prep_sources(h);
     if (rc) return rc;
     ion::FieldArgs f{};
     f.e_stat = h->e_stat;
@@ -261,7 +266,8 @@ int field_update(ion_handle* h) {
 }
 
 ion::sur::Fields sur_fields(ion_handle* h) {
-    return ion::sur::Fields{(const float4*)h->qu_lod, h->flags, h->e_stat, h->b_stat, h->e_dyn, h->b_dyn, h->field_raw};
+    return ion::sur::Fields{(const float4*)h->qu_lod, h->flags, h->e_stat, h->b_stat, h->e_dyn, h->b_dyn,
+                            h->field_raw, h->src_pq, h->src_vd, h->src_count};
 }
 
 // The field step of a time step: the direct kernel, or the surrogate with the
@@ -289,8 +295,14 @@ int field_step(ion_handle* h) {
         }
         return ION_OK;
     }
-    rc = timed(h, 3, [&] { return h->net->infer(sur_fields(h)) ? hipErrorUnknown : hipSuccess; });
-    if (rc) return fail(ION_ERR_DEVICE, "surrogate: " + h->net->error());
+    if ((rc = prep_sources(h))) return rc;
+    bool ok = true;
+    rc = timed(h, 3, [&] {
+        ok = h->net->infer(sur_fields(h)) == 0;
+        return hipSuccess;
+    });
+    if (rc) return rc;
+    if (!ok) return fail(ION_ERR_DEVICE, "surrogate: " + h->net->error());
     h->sstats.surrogate_steps += 1;
     return ION_OK;
 }
@@ -703,7 +715,7 @@ static int need_net(ion_handle* h) {
 
 int ion_surrogate_infer(ion_handle* h) {
     int rc = need_net(h);
-    if (rc) return rc;
+    if (rc || (rc = prep_sources(h))) return rc;
     if (h->net->infer(sur_fields(h))) return fail(ION_ERR_DEVICE, "surrogate: " + h->net->error());
     HIPCHK(hipStreamSynchronize(h->stream));
     return ION_OK;
@@ -711,7 +723,7 @@ int ion_surrogate_infer(ion_handle* h) {
 
 int ion_surrogate_train_step(ion_handle* h, int update, double* rel_e, double* rel_b) {
     int rc = need_net(h);
-    if (rc) return rc;
+    if (rc || (rc = prep_sources(h))) return rc;
     double re = 0, rb = 0;
     if (h->net->train_step(sur_fields(h), update != 0, &re, &rb)) return fail(ION_ERR_DEVICE, "surrogate: " + h->net->error());
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -723,7 +735,7 @@ int ion_surrogate_train_step(ion_handle* h, int update, double* rel_e, double* r
 
 int ion_surrogate_residual(ion_handle* h, double* rel_e, double* rel_b) {
     int rc = need_net(h);
-    if (rc) return rc;
+    if (rc || (rc = prep_sources(h))) return rc;
     double re = 0, rb = 0;
     if (h->net->residual(sur_fields(h), &re, &rb)) return fail(ION_ERR_DEVICE, "surrogate: " + h->net->error());
     if (rel_e) *rel_e = re;

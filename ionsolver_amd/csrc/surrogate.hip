@@ -104,6 +104,136 @@ __global__ void __launch_bounds__(256) k_gemm_nt(const __bf16* __restrict__ A, c
 }
 
 // ---------------------------------------------------------------------------
+// Far field: one workgroup per top-level LOD block. Thread t owns row
+// r = t % 9 of the block's 3^3 sample lattice (3 points along x that share
+// r_y, r_z) and the source chunk c = t / 9 (every 28th source); the sources
+// are staged through LDS in tiles and the 28 chunk partials are summed in
+// LDS in chunk order (deterministic). Interaction arithmetic: the FAST form
+// of the direct kernel (r / |r|^3 with the hardware reciprocal square root).
+// Sample k of an axis sits at ((k + 1/2) / 3) b - 1/2 in the block's cell
+// coordinates.
+// ---------------------------------------------------------------------------
+constexpr uint32_t FAR_TILE = 512, FAR_ROWS = 9, FAR_CHUNKS = 28, FAR_THREADS = 256;
+
+__global__ void __launch_bounds__(FAR_THREADS) k_far_samples(FarArgs a) {
+    __shared__ float4 s_pq[FAR_TILE];
+    __shared__ float4 s_vd[FAR_TILE];
+    __shared__ float part[FAR_CHUNKS][FAR_ROWS * 18];
+    const uint32_t tid = threadIdx.x, blk = blockIdx.x;
+    const uint32_t row = tid % FAR_ROWS, chunk = tid / FAR_ROWS;  // chunk == FAR_CHUNKS: idle lanes
+    const uint32_t mask = a.nd - 1u;
+    const int bx = (int)(blk & mask), by = (int)((blk >> a.D) & mask), bz = (int)(blk >> (2 * a.D));
+    float px[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) px[k] = (float)(bx * (int)a.lbx) + ((float)k + 0.5f) * (float)a.lbx / 3.0f - 0.5f;
+    const float py = (float)(by * (int)a.lby) + ((float)(row % 3) + 0.5f) * (float)a.lby / 3.0f - 0.5f;
+    const float pz = (float)(bz * (int)a.lbz) + ((float)(row / 3) + 0.5f) * (float)a.lbz / 3.0f - 0.5f;
+    float acc[3][6];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int c = 0; c < 6; c++) acc[k][c] = 0.0f;
+    const uint32_t count = *a.src_count;
+    for (uint32_t base = 0; base < count; base += FAR_TILE) {
+        const uint32_t m = min(FAR_TILE, count - base);
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += FAR_THREADS) {
+            s_pq[i] = a.src_pq[base + i];
+            s_vd[i] = a.src_vd[base + i];
+        }
+        __syncthreads();
+        if (chunk >= FAR_CHUNKS) continue;
+        for (uint32_t s = chunk; s < m; s += FAR_CHUNKS) {
+            const float4 vd = s_vd[s];
+            const uint32_t d = __float_as_uint(vd.w);
+            if (d < a.nblk) {  // in-domain source: skip the 3^3 neighbourhood (own block included)
+                const int dx = (int)(d & mask) - bx, dy = (int)((d >> a.D) & mask) - by, dz = (int)(d >> (2 * a.D)) - bz;
+                if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
+            }
+            const float4 pq = s_pq[s];
+            const float ry = py - pq.y, rz = pz - pq.z;
+            const float syz = __builtin_fmaf(ry, ry, rz * rz);
+            const float cx = __builtin_fmaf(vd.y, rz, -(vd.z * ry));  // (v x r).x
+            const float m1 = vd.x * rz, m2 = vd.x * ry;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float rx = px[k] - pq.x;
+                const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, syz));
+                const float t = pq.w * (inv * inv * inv);
+                acc[k][0] = __builtin_fmaf(t, rx, acc[k][0]);
+                acc[k][1] = __builtin_fmaf(t, ry, acc[k][1]);
+                acc[k][2] = __builtin_fmaf(t, rz, acc[k][2]);
+                acc[k][3] = __builtin_fmaf(t, cx, acc[k][3]);
+                acc[k][4] = __builtin_fmaf(t, __builtin_fmaf(vd.z, rx, -m1), acc[k][4]);   // (v x r).y
+                acc[k][5] = __builtin_fmaf(t, __builtin_fmaf(-vd.y, rx, m2), acc[k][5]);   // (v x r).z
+            }
+        }
+    }
+    if (chunk < FAR_CHUNKS) {
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int c = 0; c < 6; c++) part[chunk][row * 18 + k * 6 + c] = acc[k][c];
+    }
+    __syncthreads();
+    if (tid < FAR_ROWS * 18) {  // tid = row * 18 + k * 6 + c  ->  sample p = k + 3 * row
+        float sum = 0.0f;
+        for (uint32_t c = 0; c < FAR_CHUNKS; c++) sum += part[c][tid];
+        const uint32_t r = tid / 18, k = (tid / 6) % 3, comp = tid % 6;
+        a.far[((size_t)blk * FAR_PTS + k + 3 * r) * 6 + comp] = sum;
+    }
+}
+
+// Lagrange basis of the three sample positions t_k = ((k + 1/2)/3) b - 1/2
+// (spacing h = b/3) at offset u.
+__device__ __forceinline__ void lagrange3(float u, float b, float L[3]) {
+    const float h = b / 3.0f, t0 = 0.5f * h - 0.5f;
+    const float s = (u - t0) / h;  // sample coordinate: nodes at 0, 1, 2
+    L[0] = 0.5f * (s - 1.0f) * (s - 2.0f);
+    L[1] = -s * (s - 2.0f);
+    L[2] = 0.5f * s * (s - 1.0f);
+}
+
+__global__ void __launch_bounds__(256) k_far_interp(InterpArgs a) {
+    const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= a.ncell) return;
+    const bool train = a.y != nullptr;
+    if (!train && (a.flags[n] & 0x1F) == 0x01) return;
+    const uint32_t z = (uint32_t)(n / ((uint64_t)a.nx * a.ny)), r = (uint32_t)(n % ((uint64_t)a.nx * a.ny));
+    const uint32_t y = r / a.nx, x = r % a.nx;
+    const uint32_t blk = x / a.lbx + (y / a.lby + (z / a.lbz) * a.nd) * a.nd;
+    float Lx[3], Ly[3], Lz[3];
+    lagrange3((float)(x % a.lbx), (float)a.lbx, Lx);
+    lagrange3((float)(y % a.lby), (float)a.lby, Ly);
+    lagrange3((float)(z % a.lbz), (float)a.lbz, Lz);
+    const float* f = a.far + (size_t)blk * FAR_PTS * 6;
+    float acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float w = Lz[i] * Ly[j] * Lx[k];
+                const float* fp = f + (i * 9 + j * 3 + k) * 6;
+#pragma unroll
+                for (int c = 0; c < 6; c++) acc[c] = __builtin_fmaf(w, fp[c], acc[c]);
+            }
+    const uint64_t N = a.ncell;
+    if (train) {
+        const float tg = a.gain / *a.in_scale;
+#pragma unroll
+        for (int c = 0; c < 6; c++) a.y[(size_t)c * N + n] += acc[c] * tg;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            a.e_dyn[(size_t)c * N + n] += a.ke * acc[c];
+            a.b_dyn[(size_t)c * N + n] += a.kmu * acc[3 + c];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Network input: normalised (q, J) of the top-level LODs as bf16 hi + lo
 // parts (8 channels), zero for the LOD indices below `lo` that the direct
 // field update never reads (sim_kernels.cl:943).
@@ -351,6 +481,14 @@ hipError_t gemm_nt(int epi, const __bf16* A, const __bf16* B, int M, int N, int 
     if (epi == EPI_F32) hipLaunchKernelGGL(k_gemm_nt<EPI_F32>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
     else if (epi == EPI_BF16) hipLaunchKernelGGL(k_gemm_nt<EPI_BF16>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
     else hipLaunchKernelGGL(k_gemm_nt<EPI_HEAD>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
+    return hipGetLastError();
+}
+hipError_t far_samples(const FarArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_far_samples, dim3(a.nblk), dim3(FAR_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t far_interp(const InterpArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_far_interp, dim3((uint32_t)((a.ncell + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 hipError_t input(const float4* qu, uint32_t n_top, uint32_t lo, float* scale, __bf16* x, hipStream_t s) {

@@ -36,6 +36,35 @@ struct HeadGeom {
     uint32_t ld_dy, ld_dyt;  // leading dimensions of dY [block][ld_dy] and dYt [n][ld_dyt]
 };
 
+// Far field of a top-level LOD block: the sources outside its 3^3 block
+// neighbourhood (the ones the head's near-field stencil does not see),
+// sampled at a 3^3 lattice of points inside the block.
+constexpr int FAR_PTS = 27;
+struct FarArgs {
+    const float4* src_pq;      // compacted LOD sources (position, charge)
+    const float4* src_vd;      // (velocity, LOD index bits)
+    const uint32_t* src_count;
+    float* far;                // [block][27][6]: E then B raw sums
+    uint32_t nd, D, nblk;
+    uint32_t lbx, lby, lbz;    // cells per block side
+};
+// Tri-quadratic interpolation of the far samples onto the cells, added to
+// E_dyn / B_dyn (k = KE, KMU; fluid cells) or, when y != null, to the
+// training output y [6][N] in network units (times gain / s).
+struct InterpArgs {
+    const float* far;
+    float* e_dyn;
+    float* b_dyn;
+    const uint8_t* flags;
+    float* y;
+    const float* in_scale;
+    float ke, kmu, gain;
+    uint64_t ncell;
+    uint32_t nx, ny, nd, lbx, lby, lbz;
+};
+
+hipError_t far_samples(const FarArgs& a, hipStream_t s);
+hipError_t far_interp(const InterpArgs& a, hipStream_t s);
 hipError_t gemm_nt(int epi, const __bf16* A, const __bf16* B, int M, int N, int K, int lda, int ldb, const GemmOut& o,
                    hipStream_t s);
 hipError_t input(const float4* qu, uint32_t n_top, uint32_t lo, float* scale, __bf16* x, hipStream_t s);

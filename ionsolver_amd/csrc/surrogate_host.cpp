@@ -11,10 +11,15 @@
 //   e2  = conv(pool(e1))     C2   g2        d0 = conv([up(d1), e0])  C0   g0
 //   e3  = conv(pool(e2))     C3   g3
 //   y   = subpixel conv([d0, x]): 6 field components for every cell of the
-//         LOD block, E = E_stat + KE s y_E, B = B_stat + KMU s y_B
-// with g_l = nd / 2^l, C_l = c0 (1, 2, 3, 4). The head starts at the exact
-// near field (the 26 neighbouring LOD sources, the own block excluded, as in
-// update_e_b_dynamic); everything else is learned.
+//         LOD block
+// with g_l = nd / 2^l, C_l = c0 (1, 2, 3, 4), plus the global path: the far
+// field of each block (all sources outside its 3^3 block neighbourhood),
+// summed directly at a 3^3 lattice of points per block and interpolated
+// tri-quadratically onto the cells. E = E_stat + KE (s y_E / gain + far_E),
+// B likewise with KMU. The head starts at the exact near field (the 26
+// neighbouring LOD sources, the own block excluded, as in
+// update_e_b_dynamic) and zero weights on the U-Net features; training
+// fits the U-Net and the head to what the near + far split leaves.
 #include "surrogate_host.hpp"
 
 #include <cmath>
@@ -107,7 +112,8 @@ int Net::create(const Geom& geo, const Config& cfg, hipStream_t stream) {
         (rc = add_layer(C[1], C[0], C[0], g0, &layers_[D0])))
         return rc;
     // pooled / upsampled activations and their gradients
-    if ((rc = alloc((void**)&x_, (size_t)g0 * g0 * g0 * 8 * 2)) || (rc = alloc((void**)&scale_, 256)))
+    if ((rc = alloc((void**)&x_, (size_t)g0 * g0 * g0 * 8 * 2)) || (rc = alloc((void**)&scale_, 256)) ||
+        (rc = alloc((void**)&far_, (size_t)g0 * g0 * g0 * FAR_PTS * 6 * 4)))
         return rc;
     for (int l = 0; l < 3; l++) {
         const int g = g0 >> l, h = g / 2, c = C[l];
@@ -191,13 +197,10 @@ int Net::init_weights() {
         SCHK(hipMemcpyAsync(L.w32, w.data(), w.size() * 4, hipMemcpyHostToDevice, stream_));
         SCHK(hipStreamSynchronize(stream_));
     }
-    // head: exact near field on the source channels, small random weights on
-    // the level-0 feature channels (so that the U-Net receives gradients)
+    // head: exact near field on the source channels, zero on the U-Net
+    // features (their gradient is not zero, so the U-Net starts learning as
+    // soon as the head's feature columns move)
     std::vector<float> hw((size_t)nout_ * hk_, 0.0f);
-    std::normal_distribution<float> hn(0.0f, 0.1f / std::sqrt(27.0f * cfg_.c0));
-    for (uint32_t r = 0; r < nout_; r++)
-        for (int d = 0; d < 27; d++)
-            for (int ci = 0; ci < cfg_.c0; ci++) hw[(size_t)r * hk_ + d * hcin_ + ci] = hn(rng);
     near_field_head(hw);
     SCHK(hipMemcpyAsync(hw32_, hw.data(), hw.size() * 4, hipMemcpyHostToDevice, stream_));
     SCHK(hipStreamSynchronize(stream_));
@@ -271,11 +274,41 @@ GemmOut Net::head_out(const Fields& f, float* fine) const {
     return o;
 }
 
+// Far-field samples of every block, interpolated onto the cells: added to
+// E_dyn / B_dyn (y == null) or to the training output y in network units.
+int Net::far_field(const Fields& f, float* y) {
+    if (f.src_pq == nullptr || f.src_vd == nullptr || f.src_count == nullptr) {
+        err_ = "no LOD sources for the far field";
+        return -2;
+    }
+    FarArgs fa{f.src_pq, f.src_vd, f.src_count, far_, nd_, geo_.D, nblk_, geo_.nx / nd_, geo_.ny / nd_, geo_.nz / nd_};
+    SCHK(far_samples(fa, stream_));
+    InterpArgs ia{};
+    ia.far = far_;
+    ia.e_dyn = f.e_dyn;
+    ia.b_dyn = f.b_dyn;
+    ia.flags = f.flags;
+    ia.y = y;
+    ia.in_scale = scale_;
+    ia.ke = geo_.ke;
+    ia.kmu = geo_.kmu;
+    ia.gain = gain_;
+    ia.ncell = ncell_;
+    ia.nx = geo_.nx;
+    ia.ny = geo_.ny;
+    ia.nd = nd_;
+    ia.lbx = geo_.nx / nd_;
+    ia.lby = geo_.ny / nd_;
+    ia.lbz = geo_.nz / nd_;
+    SCHK(far_interp(ia, stream_));
+    return 0;
+}
+
 int Net::infer(const Fields& f) {
     int rc = forward_features(f.qu);
     if (rc) return rc;
     SCHK(gemm_nt(EPI_HEAD, hcol_, hw_, (int)nblk_, (int)nout_, hk_, hk_, hk_, head_out(f, nullptr), stream_));
-    return 0;
+    return far_field(f, nullptr);
 }
 
 int Net::loss(const Fields& f, double* rel_e, double* rel_b) {
@@ -301,6 +334,7 @@ int Net::residual(const Fields& f, double* rel_e, double* rel_b) {
     int rc;
     if ((rc = ensure_training_buffers()) || (rc = forward_features(f.qu))) return rc;
     SCHK(gemm_nt(EPI_HEAD, hcol_, hw_, (int)nblk_, (int)nout_, hk_, hk_, hk_, head_out(f, y_), stream_));
+    if ((rc = far_field(f, y_))) return rc;
     return loss(f, rel_e, rel_b);
 }
 
@@ -367,7 +401,7 @@ int Net::train_step(const Fields& f, bool update, double* rel_e, double* rel_b) 
     int rc;
     if ((rc = ensure_training_buffers()) || (rc = forward_features(f.qu))) return rc;
     SCHK(gemm_nt(EPI_HEAD, hcol_, hw_, (int)nblk_, (int)nout_, hk_, hk_, hk_, head_out(f, y_), stream_));
-    if ((rc = loss(f, rel_e, rel_b))) return rc;
+    if ((rc = far_field(f, y_)) || (rc = loss(f, rel_e, rel_b))) return rc;
     SCHK(hipMemcpyAsync(norms_, norm_host_, 2 * sizeof(float), hipMemcpyHostToDevice, stream_));
     HeadGeom hg{ncell_, geo_.nx, geo_.ny, geo_.nz, nd_, geo_.nx / nd_, geo_.ny / nd_, geo_.nz / nd_, cpb_,
                 (uint32_t)nout_pad_, (uint32_t)nblk_pad_};

@@ -36,6 +36,9 @@ struct Fields {
     float* e_dyn;
     float* b_dyn;
     const float* target;
+    const float4* src_pq;  // compacted LOD sources of the direct kernel (prep_sources)
+    const float4* src_vd;
+    const uint32_t* src_count;
 };
 
 class Net {
@@ -78,6 +81,7 @@ class Net {
     int ensure_training_buffers();
     void near_field_head(std::vector<float>& w) const;
     GemmOut head_out(const Fields& f, float* fine) const;
+    int far_field(const Fields& f, float* y);
 
     Geom geo_{};
     Config cfg_{};
@@ -88,6 +92,7 @@ class Net {
     float gain_ = 1.0f;  // network output = raw field sum * gain / s (gain = block side^2)
     Layer layers_[NLAYER];
     __bf16* x_ = nullptr;
+    float* far_ = nullptr;  // far-field samples [block][27][6]
     float* scale_ = nullptr;
     __bf16* pool_[3] = {nullptr, nullptr, nullptr};
     float* dpool_[3] = {nullptr, nullptr, nullptr};

@@ -6,9 +6,13 @@ input (q, J = q v) of the top-level LOD blocks / s as bf16 hi + lo parts,
 a linear 4-level 3D conv U-Net over the LOD grid (3^3 convolutions, 2^3
 average pooling, nearest upsampling, skip concatenation), and the sub-pixel
 output head that maps the 3^3 neighbourhood of [d0, x] to the 6 field
-components of every cell of a LOD block. Loss: relative squared error of E
-and of B over fluid cells. The backward pass is written out by hand; the
-tests check it against central finite differences.
+components of every cell of a LOD block; plus the global path: far-field
+samples of every block (direct sums over the sources outside its 3^3
+neighbourhood at a 3^3 point lattice) and their tri-quadratic
+interpolation onto the cells. Loss: relative squared error of E and of B
+over fluid cells. The backward pass is written out by hand; the tests check
+it against central finite differences, the far field against the direct
+field of the C oracle.
 
 Parameter layout (flat): layers e0, e1, e2, e3, d2, d1, d0, head, each a
 [cout][kpad] matrix with column k = delta * cin + ci, delta = (dx+1) +
@@ -171,11 +175,66 @@ def targets(geo, s, raw):
     return raw.reshape(6, -1).astype(np.float64) * (geo.gain / s)
 
 
-def loss_and_grad(geo, params, qu, T, fluid):
+def far_samples(geo, qu):
+    """Far field of every top-level block: the in-range sources (LOD index
+    >= lo, q != 0) outside the block's 3^3 neighbourhood, summed directly
+    (E from q, B from q v x r, r / |r|^3) at the 3^3 lattice of points
+    ((k + 1/2)/3) b - 1/2 of each axis. Sources sit at the LOD centres
+    ((i + 1/2) b) in cell coordinates (sim_kernels.cl:442-447). Returns
+    [nblk][27][6] float64 (E then B)."""
+    nd, D = geo.nd, geo.D
+    q = qu[:geo.nblk].astype(np.float64)
+    src = np.arange(geo.lo, geo.nblk)
+    src = src[q[src, 0] != 0.0]
+    b = np.array([geo.bx, geo.by, geo.bz], np.float64)
+    sc = np.stack([src % nd, (src // nd) % nd, src // (nd * nd)], 1)
+    spos = (sc + 0.5) * b
+    blk = np.arange(geo.nblk)
+    bc = np.stack([blk % nd, (blk // nd) % nd, blk // (nd * nd)], 1)
+    k = np.arange(27)
+    pk = np.stack([k % 3, (k // 3) % 3, k // 9], 1)
+    pts = bc[:, None, :] * b + ((pk[None, :, :] + 0.5) / 3.0) * b - 0.5       # [nblk][27][3]
+    out = np.zeros((geo.nblk, 27, 6))
+    for j, sidx in enumerate(src):
+        far = np.any(np.abs(bc - sc[j]) > 1, axis=1)                          # [nblk]
+        r = pts - spos[j]
+        r3 = np.sum(r * r, axis=2) ** 1.5
+        t = np.where(far[:, None], q[sidx, 0] / r3, 0.0)
+        v = q[sidx, 1:4]
+        out[:, :, 0:3] += t[..., None] * r
+        out[:, :, 3:6] += t[..., None] * np.cross(v, r)
+    return out
+
+
+def lagrange3(u, b):
+    """Quadratic Lagrange basis of the nodes ((k + 1/2)/3) b - 1/2, k = 0, 1, 2."""
+    h = b / 3.0
+    s = (np.asarray(u, np.float64) - (0.5 * h - 0.5)) / h
+    return np.stack([0.5 * (s - 1) * (s - 2), -s * (s - 2), 0.5 * s * (s - 1)])
+
+
+def far_interp(geo, samples):
+    """Tri-quadratic interpolation of far samples onto every cell: (6, N)."""
+    n = np.arange(geo.nx * geo.ny * geo.nz)
+    x, y, z = n % geo.nx, (n // geo.nx) % geo.ny, n // (geo.nx * geo.ny)
+    blk = x // geo.bx + geo.nd * (y // geo.by + geo.nd * (z // geo.bz))
+    Lx, Ly, Lz = lagrange3(x % geo.bx, geo.bx), lagrange3(y % geo.by, geo.by), lagrange3(z % geo.bz, geo.bz)
+    out = np.zeros((6, n.size))
+    for p in range(27):
+        w = Lz[p // 9] * Ly[(p // 3) % 3] * Lx[p % 3]
+        out += w[None, :] * samples[blk, p, :].T
+    return out
+
+
+def loss_and_grad(geo, params, qu, T, fluid, y_add=None):
     """Loss sum_E (y-t)^2/sum_E t^2 + sum_B (y-t)^2/sum_B t^2 over fluid cells,
-    its gradient w.r.t. the flat parameters, and (rel_e, rel_b)."""
+    its gradient w.r.t. the flat parameters, and (rel_e, rel_b). y_add: a
+    constant (6, N) term added to the network output (the far field in
+    network units)."""
     Y, s, a = forward(geo, params, qu)
     y = block_to_cells(geo, Y)
+    if y_add is not None:
+        y = y + y_add
     m = fluid[None, :]
     d = (y - T) * m
     nE, nB = np.sum((T[:3] * m) ** 2), np.sum((T[3:] * m) ** 2)
