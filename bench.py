@@ -95,6 +95,42 @@ def cpu_baseline(name, seconds_hint):
                       + ", oracle built -O3 -fopenmp"}
 
 
+def surrogate_leg(lbm, cfg, samples=4, reps=10, channels=16):
+    """Learned field surrogate against the direct field step on the same
+    lattice: accuracy on `samples` random charge configurations (the
+    surrogate's own relative L2 residual of the dynamic E and B against the
+    direct kernel's raw sums) and the field-step time of both (library
+    timing slots: source compaction + field kernel(s), HIP events on the
+    library stream)."""
+    from ionsolver_amd.scenarios import random_lod
+    from ionsolver_amd.surrogate import lod_buffer
+    from ionsolver_amd import Field
+    lbm.surrogate_create(channels=channels)
+    rng = np.random.default_rng(777)
+    shape, D = (cfg.n_x, cfg.n_y, cfg.n_z), cfg.mhd_lod_depth
+    res = []
+    for _ in range(samples):
+        lbm.write_field(Field.QU_LOD, lod_buffer(lbm, random_lod(shape, D, rng)))
+        lbm.compute_fields()
+        res.append(lbm.surrogate_residual())
+    res = np.array(res)
+
+    def field_ms(fn):
+        fn()
+        lbm.set_timing(True)
+        for _ in range(reps):
+            fn()
+        t = lbm.get_timing()
+        lbm.set_timing(False)
+        return (t["ms_field_prep"] + t["ms_update_e_b"]) / reps
+
+    d_ms, s_ms = field_ms(lbm.compute_fields), field_ms(lbm.surrogate_infer)
+    return {"field_step_ms_direct": round(d_ms, 4), "field_step_ms_surrogate": round(s_ms, 4),
+            "speedup": round(d_ms / s_ms, 2), "rel_l2_e": float(res[:, 0].max()), "rel_l2_b": float(res[:, 1].max()),
+            "channels": channels, "train_steps": 0,
+            "accuracy_sample": f"{samples} random 8-blob LOD configurations on this lattice (max over samples)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +139,7 @@ def main():
     ap.add_argument("--workload", default="mhd256")
     ap.add_argument("--field-mode", default="fast", choices=["exact", "fast"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned field surrogate leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate", action="store_true",
                     help="first run the device FP16C decode on 2^26 codes (reads 128 MiB, writes 256 MiB): "
@@ -181,6 +218,8 @@ def main():
                                         "field_mode": args.field_mode, "fluid_cells": n_f, **meta},
         "roofline": roofline, "kernels": kernels,
     }
+    if cfg.ext_magneto_hydro and not args.no_surrogate and cfg.mhd_lod_depth >= 3:
+        out["surrogate"] = surrogate_leg(lbm, cfg)
     lbm.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)

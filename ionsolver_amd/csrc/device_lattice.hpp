@@ -152,25 +152,29 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fm
 // DDF storage formats (load/store macros, src/lbm/domain.rs:773-784)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint16_t fp16c_encode(float x) {
-    // custom 1-4-11 format, bias 15 (sim_kernels.cl:79-84)
-    // a NaN of either sign is encoded as the positive quiet NaN (0/0 on the GPU)
+    // custom 1-4-11 format, bias 15 (sim_kernels.cl:79-84). With b = bits +
+    // 0x800 (the reference's rounding add) and e its exponent field:
+    //  - e >= 113: code bits 14..0 = (b >> 12) & 0x7FFF (the reference's
+    //    ((e - 112) << 11 & 0x7800) | m >> 12, since 112 is a multiple of 16);
+    //  - e <= 112: the reference's ((M >> (124 - e)) + 1) >> 1 on the 24-bit
+    //    significand M of x equals floor(|x| 2^25 + 1/2) (and 0 for e <= 100),
+    //    computed as (floor(|x| 2^26) + 1) >> 1: the scaling is exact and the
+    //    conversion truncates (adding 1/2 in float would round up at
+    //    |x| 2^25 = 1/2 - 2^-25).
+    // A NaN of either sign is encoded as the positive quiet NaN (0/0 on the GPU).
     const uint32_t b = (x != x ? 0x7FC00000u : __float_as_uint(x)) + 0x00000800u;
-    const uint32_t e = (b & 0x7F800000u) >> 23;
-    const uint32_t m = b & 0x007FFFFFu;
-    const uint32_t sgn = (b & 0x80000000u) >> 16;
-    const uint32_t nrm = (((e - 112u) << 11) & 0x7800u) | (m >> 12);
-    const uint32_t den = (((0x007FF800u + m) >> ((124u - e) & 31u)) + 1u) >> 1;
-    // branch-free selection of the normal / denormal / flushed case
-    const uint32_t mn = 0u - (uint32_t)(e > 112u);
-    const uint32_t md = 0u - (uint32_t)((e < 113u) & (e > 100u));
-    return (uint16_t)(sgn | (nrm & mn) | (den & md));
+    const uint32_t nrm = (b >> 12) & 0x7FFFu;
+    const uint32_t den = ((uint32_t)(__builtin_fabsf(x) * 0x1p26f) + 1u) >> 1;
+    const uint32_t mag = (b & 0x7FFFFFFFu) >= 0x38800000u ? nrm : den;
+    return (uint16_t)(((b >> 16) & 0x8000u) | mag);
 }
 __device__ __forceinline__ float fp16c_decode(uint16_t x) {
-    // sim_kernels.cl:85-90; denormal codes are m * 2^-25 (exact)
-    const uint32_t e = (x & 0x7800u) >> 11;
-    const uint32_t m = x & 0x07FFu;
-    const float v = e != 0u ? __uint_as_float(((e + 112u) << 23) | (m << 12)) : (float)m * 0x1p-25f;
-    return (x & 0x8000u) ? -v : v;
+    // sim_kernels.cl:85-90. Bits 14..0 of a code placed at bits 26..12 of a
+    // float give the value times 2^-112 (normal codes as normal floats,
+    // denormal codes as denormal floats, which the kernels keep), so one
+    // exact multiplication by 2^112 decodes every code.
+    const uint32_t u = (uint32_t)x;
+    return __uint_as_float(((u & 0x7FFFu) << 12) | ((u & 0x8000u) << 16)) * 0x1p112f;
 }
 
 struct F32 {

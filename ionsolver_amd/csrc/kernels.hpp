@@ -28,6 +28,9 @@ struct LbmArgs {
     uint32_t nx, ny, nz;
     uint32_t pad0;
     uint64_t N;
+    uint64_t N_store;     // == N; a separate argument so that the compiler computes the
+                          // store-side slot bases at the stores instead of keeping the
+                          // load-side ones live in scalar registers across the cell
     uint32_t odd;         // t % 2 (esoteric-pull layout)
     uint32_t eqb, vforce, ffield, upd;  // extension switches
     uint32_t lod_bx;      // LOD block side along x (cells)

@@ -45,21 +45,36 @@ template <int Q, int I> __device__ __forceinline__ uint32_t nb_index(const Nbr& 
     return x + y + z;
 }
 
+// Element idx of the array starting at `base` through a 32-bit byte offset,
+// which gives the global load/store form with a scalar 64-bit base and a
+// 32-bit vector offset (one VALU shift per distinct index instead of 64-bit
+// address arithmetic per access). Valid while idx * sizeof(T) < 2^32: every
+// array here has at most N <= 2^30 elements of at most 4 bytes per base
+// (the host rejects larger lattices).
+template <class T> __device__ __forceinline__ T* at(T* base, uint32_t idx) {
+    return (T*)((char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <class T> __device__ __forceinline__ const T* at(const T* base, uint32_t idx) {
+    return (const T*)((const char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+
 // Esoteric-pull load/store of an NQ-direction lattice (load_f/store_f and
 // load_a/store_a, sim_kernels.cl:234-247, 397-410). Direction pair (i, i+1)
 // of cell n lives in slots (n, i+1) and (j[i], i) on even steps and in
-// (n, i) and (j[i], i+1) on odd steps.
+// (n, i) and (j[i], i+1) on odd steps. Slot bases are uniform (odd is a
+// kernel argument, 0 or 1, used arithmetically so that the compiler keeps
+// the slot selection on the scalar unit); only the cell index varies per lane.
 template <int Q, int NQ, class S>
 __device__ __forceinline__ void load_ddfs(const typename S::T* __restrict__ buf, uint64_t N, uint32_t n,
                                           const Nbr& b, uint32_t odd, float* fh) {
-    fh[0] = S::ld(buf[n]);
+    fh[0] = S::ld(*at(buf, n));
     static_for<(NQ - 1) / 2>([&](auto p) {
         constexpr int i = 2 * decltype(p)::value + 1;
         const uint32_t j = nb_index<Q, i>(b);
-        const uint64_t so = odd ? (uint64_t)i : (uint64_t)(i + 1);
-        const uint64_t sn = odd ? (uint64_t)(i + 1) : (uint64_t)i;
-        fh[i] = S::ld(buf[so * N + n]);
-        fh[i + 1] = S::ld(buf[sn * N + j]);
+        const typename S::T* so = buf + (uint64_t)(i + 1u - odd) * N;
+        const typename S::T* sn = buf + (uint64_t)(i + odd) * N;
+        fh[i] = S::ld(*at(so, n));
+        fh[i + 1] = S::ld(*at(sn, j));
     });
 }
 // Raw (undecoded) esoteric-pull loads, so that a kernel can issue the loads
@@ -67,14 +82,14 @@ __device__ __forceinline__ void load_ddfs(const typename S::T* __restrict__ buf,
 template <int Q, int NQ, class S>
 __device__ __forceinline__ void load_raw(const typename S::T* __restrict__ buf, uint64_t N, uint32_t n, const Nbr& b,
                                          uint32_t odd, typename S::T* raw) {
-    raw[0] = buf[n];
+    raw[0] = *at(buf, n);
     static_for<(NQ - 1) / 2>([&](auto p) {
         constexpr int i = 2 * decltype(p)::value + 1;
         const uint32_t j = nb_index<Q, i>(b);
-        const uint64_t so = odd ? (uint64_t)i : (uint64_t)(i + 1);
-        const uint64_t sn = odd ? (uint64_t)(i + 1) : (uint64_t)i;
-        raw[i] = __builtin_nontemporal_load(&buf[so * N + n]);
-        raw[i + 1] = __builtin_nontemporal_load(&buf[sn * N + j]);
+        const typename S::T* so = buf + (uint64_t)(i + 1u - odd) * N;
+        const typename S::T* sn = buf + (uint64_t)(i + odd) * N;
+        raw[i] = __builtin_nontemporal_load(at(so, n));
+        raw[i + 1] = __builtin_nontemporal_load(at(sn, j));
     });
 }
 template <int NQ, class S>
@@ -86,14 +101,14 @@ __device__ __forceinline__ void decode(const typename S::T* raw, float* fh) {
 template <int Q, int NQ, class S>
 __device__ __forceinline__ void store_ddfs(typename S::T* __restrict__ buf, uint64_t N, uint32_t n, const Nbr& b,
                                            uint32_t odd, const float* fh) {
-    buf[n] = S::st(fh[0]);
+    *at(buf, n) = S::st(fh[0]);
     static_for<(NQ - 1) / 2>([&](auto p) {
         constexpr int i = 2 * decltype(p)::value + 1;
         const uint32_t j = nb_index<Q, i>(b);
-        const uint64_t so = odd ? (uint64_t)i : (uint64_t)(i + 1);
-        const uint64_t sn = odd ? (uint64_t)(i + 1) : (uint64_t)i;
-        __builtin_nontemporal_store(S::st(fh[i]), &buf[sn * N + j]);
-        __builtin_nontemporal_store(S::st(fh[i + 1]), &buf[so * N + n]);
+        typename S::T* so = buf + (uint64_t)(i + 1u - odd) * N;
+        typename S::T* sn = buf + (uint64_t)(i + odd) * N;
+        __builtin_nontemporal_store(S::st(fh[i]), at(sn, j));
+        __builtin_nontemporal_store(S::st(fh[i + 1]), at(so, n));
     });
 }
 
@@ -147,7 +162,7 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
     const uint32_t n = (uint32_t)n64;
     bool active = n64 < N;
     uint8_t bo = 0x01;
-    if (active) bo = a.flags[n] & 0x1F;
+    if (active) bo = *at(a.flags, n) & 0x1F;
     active = active && bo != 0x01;
     float4 lod = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     typedef typename S::T T;
@@ -162,17 +177,17 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
         if (MHD) {
             load_raw<Q, Q, S>((const T*)a.ei, N, n, b, a.odd, re);
             load_raw<19, 7, S>((const T*)a.fqi, N, n, b, a.odd, rq);
-            Bx = a.b_dyn[n]; By = a.b_dyn[N + n]; Bz = a.b_dyn[2 * N + n];
-            Ex = a.e_dyn[n]; Ey = a.e_dyn[N + n]; Ez = a.e_dyn[2 * N + n];
+            Bx = *at(a.b_dyn, n); By = *at(a.b_dyn + N, n); Bz = *at(a.b_dyn + 2 * N, n);
+            Ex = *at(a.e_dyn, n); Ey = *at(a.e_dyn + N, n); Ez = *at(a.e_dyn + 2 * N, n);
         }
         float fhn[Q];
         decode<Q, S>(rf, fhn);
         float rhon, uxn, uyn, uzn;
         if (is_e) {
-            rhon = a.rho[n];
-            uxn = a.u[n];
-            uyn = a.u[N + n];
-            uzn = a.u[2 * N + n];
+            rhon = *at(a.rho, n);
+            uxn = *at(a.u, n);
+            uyn = *at(a.u + N, n);
+            uzn = *at(a.u + 2 * N, n);
         } else {
             rho_u<Q>(fhn, rhon, uxn, uyn, uzn);
         }
@@ -181,9 +196,9 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
         const float w = a.w;
         const float c_tau = __builtin_fmaf(w, -0.5f, 1.0f);
         if (a.ffield) {
-            fxn += a.F[n];
-            fyn += a.F[N + n];
-            fzn += a.F[2 * N + n];
+            fxn += *at(a.F, n);
+            fyn += *at(a.F + N, n);
+            fzn += *at(a.F + 2 * N, n);
         }
         if (MHD) {
             // electron gas part 1
@@ -199,13 +214,13 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
             for (int i = 0; i < 7; i++) rhon_q += qhn[i];
             rhon_q += 1.0f;
             // charge advection part 2
-            a.q[n] = rhon_q - rhon_e;
+            *at(a.q, n) = rhon_q - rhon_e;
             float qeq[7];
             a_eq(rhon_q, uxn, uyn, uzn, qeq);
             const float wq = a.wq;
 #pragma unroll
             for (int i = 0; i < 7; i++) qhn[i] = __builtin_fmaf(1.0f - wq, qhn[i], wq * qeq[i]);
-            store_ddfs<19, 7, S>((T*)a.fqi, N, n, b, a.odd, qhn);
+            store_ddfs<19, 7, S>((T*)a.fqi, a.N_store, n, b, a.odd, qhn);
             // electron gas part 2: F = -rho_e (E + u_e x B), Guo forcing, SRT
             const float crx = uyn_e * Bz - uzn_e * By;
             const float cry = uzn_e * Bx - uxn_e * Bz;
@@ -222,7 +237,7 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
                 const float fi = Fin[i] * c_tau;
                 ehn[i] = is_e ? feq[i] : __builtin_fmaf(1.0f - w, ehn[i], __builtin_fmaf(w, feq[i], fi));
             }
-            store_ddfs<Q, Q, S>((T*)a.ei, N, n, b, a.odd, ehn);
+            store_ddfs<Q, Q, S>((T*)a.ei, a.N_store, n, b, a.odd, ehn);
             // EM force on the gas (plain multiply-add, as written)
             fxn += rhon_q * (Ex + uyn * Bz - uzn * By);
             fyn += rhon_q * (Ey + uzn * Bx - uxn * Bz);
@@ -243,10 +258,10 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
             for (int i = 0; i < Q; i++) Fin[i] = 0.0f;
         }
         if (a.upd && !is_e) {
-            a.rho[n] = rhon;
-            a.u[n] = uxn;
-            a.u[N + n] = uyn;
-            a.u[2 * N + n] = uzn;
+            *at(a.rho, n) = rhon;
+            *at(a.u, n) = uxn;
+            *at(a.u + N, n) = uyn;
+            *at(a.u + 2 * N, n) = uzn;
         }
         f_eq<Q>(rhon, uxn, uyn, uzn, feq);
         if (!TRT) {
@@ -287,7 +302,7 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
                               : __builtin_fmaf(0.5f * wp, feq[i] - fhn[i] + feb[i] - fhb[i],
                                                __builtin_fmaf(0.5f * wm, feq[i] - feb[i] - fhn[i] + fhb[i], fhn[i] + Fin[i]));
         }
-        store_ddfs<Q, Q, S>((T*)a.fi, N, n, b, a.odd, fhn);
+        store_ddfs<Q, Q, S>((T*)a.fi, a.N_store, n, b, a.odd, fhn);
     }
 
     if (LOD) {

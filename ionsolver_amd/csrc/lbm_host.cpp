@@ -198,6 +198,7 @@ ion::LbmArgs lbm_args(ion_handle* h) {
     a.ny = h->cfg.n_y;
     a.nz = h->cfg.n_z;
     a.N = h->N;
+    a.N_store = h->N;
     a.odd = (uint32_t)(h->t & 1u);
     a.eqb = h->cfg.ext_equilibrium_boudaries ? 1u : 0u;
     a.vforce = h->cfg.ext_volume_force ? 1u : 0u;
@@ -345,7 +346,9 @@ int validate(const ion_config& c) {
     if (c.d_x != 1 || c.d_y != 1 || c.d_z != 1)
         return fail(ION_ERR_UNSUPPORTED, "multi-domain split (d_x, d_y, d_z != 1) is replaced by a single GPU");
     const uint64_t N = (uint64_t)c.n_x * c.n_y * c.n_z;
-    if (N > 0x100000000ull) return fail(ION_ERR_INVALID, "more than 2^32 cells (the reference indexes cells with uint)");
+    if (N > 0x40000000ull)
+        return fail(ION_ERR_UNSUPPORTED, "more than 2^30 cells (1024^3) on one GPU: the kernels address every "
+                                         "per-direction array with 32-bit byte offsets");
     if (c.ext_subgrid_ecr) return fail(ION_ERR_UNSUPPORTED, "ext_subgrid_ecr is not supported");
     if (c.ext_force_field && !c.ext_volume_force) return fail(ION_ERR_INVALID, "ext_force_field needs ext_volume_force");
     if (c.ext_magneto_hydro) {
@@ -716,7 +719,13 @@ static int need_net(ion_handle* h) {
 int ion_surrogate_infer(ion_handle* h) {
     int rc = need_net(h);
     if (rc || (rc = prep_sources(h))) return rc;
-    if (h->net->infer(sur_fields(h))) return fail(ION_ERR_DEVICE, "surrogate: " + h->net->error());
+    bool ok = true;
+    rc = timed(h, 3, [&] {
+        ok = h->net->infer(sur_fields(h)) == 0;
+        return hipSuccess;
+    });
+    if (rc) return rc;
+    if (!ok) return fail(ION_ERR_DEVICE, "surrogate: " + h->net->error());
     HIPCHK(hipStreamSynchronize(h->stream));
     return ION_OK;
 }

@@ -68,6 +68,17 @@ def test_device_codecs_match_oracle():
         g, o = I.device_codec(ft, 0, x), enc(x)
         bad = np.flatnonzero(g != o)
         assert bad.size == 0, (ft, bad.size, x[bad[:8]], g[bad[:8]], o[bad[:8]])
+    # FP16C encoder around its normal / denormal / flush boundaries: every
+    # mantissa for the exponent fields 100, 101, 111, 112 and 113 (the +0x800
+    # rounding carry moves 112 into 113), a strided sweep over 96..130
+    m = np.arange(1 << 23, dtype=np.uint32)
+    full = np.concatenate([(e << 23) | m for e in (100, 101, 111, 112, 113)])
+    strided = np.concatenate([(e << 23) | m[::13] for e in range(96, 131)])
+    sweep = np.concatenate([full, strided])
+    sweep = np.concatenate([sweep, sweep | np.uint32(0x80000000)]).view(np.float32)
+    g, o = I.device_codec(1, 0, sweep), O.fp16c_encode(sweep)
+    bad = np.flatnonzero(g != o)
+    assert bad.size == 0, (bad.size, sweep[bad[:8]].view(np.uint32), g[bad[:8]], o[bad[:8]])
     for ft, dec in ((1, O.fp16c_decode), (0, O.fp16s_load)):
         g, o = I.device_codec(ft, 1, codes), dec(codes)
         assert np.array_equal(g.view(np.uint32)[~np.isnan(o)], o.view(np.uint32)[~np.isnan(o)])
