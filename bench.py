@@ -142,8 +142,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned field surrogate leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate", action="store_true",
-                    help="first run the device FP16C decode on 2^26 codes (reads 128 MiB, writes 256 MiB): "
-                         "the reference streaming kernel for PMC FETCH_SIZE/WRITE_SIZE scale factors")
+                    help="first run the HBM copy probe (3 copies of 1 GiB: reads 1 GiB, writes 1 GiB each): "
+                         "the known-size kernel for PMC FETCH_SIZE/WRITE_SIZE scale factors")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
@@ -154,7 +154,7 @@ def main():
     from ionsolver_amd import FieldMode, scenarios
 
     if args.calibrate:
-        I.device_codec(I.FloatType.FP16C, 1, np.zeros(1 << 26, np.uint16))
+        I.hbm_copy_probe(1 << 30, 2)
     cfg, arrays, meta = workload(args.workload)
     lbm = I.Lbm(cfg)
     scenarios.apply(lbm, arrays)
@@ -218,6 +218,11 @@ def main():
                                         "field_mode": args.field_mode, "fluid_cells": n_f, **meta},
         "roofline": roofline, "kernels": kernels,
     }
+    _, copy_gbs = I.hbm_copy_probe(1 << 30, 10)
+    out["hbm_copy_probe"] = {"GB/s": round(copy_gbs, 1), "frac_of_peak": round(copy_gbs / HBM_PEAK_GBS, 4),
+                             "note": "best of 10 nontemporal 16-byte copies of 1 GiB (read + write)"}
+    kernels["stream_collide"]["frac_of_copy_probe"] = round(kernels["stream_collide"]["GB/s"] / copy_gbs, 4) \
+        if kernels["stream_collide"]["GB/s"] else None
     if cfg.ext_magneto_hydro and not args.no_surrogate and cfg.mhd_lod_depth >= 3:
         out["surrogate"] = surrogate_leg(lbm, cfg)
     lbm.close()

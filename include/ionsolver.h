@@ -242,6 +242,15 @@ int ion_surrogate_param_count(ion_handle* h, uint64_t* n);
 int ion_surrogate_read_params(ion_handle* h, int grads, float* dst, uint64_t n);
 int ion_surrogate_write_params(ion_handle* h, const float* src, uint64_t n);
 
+/* ------------------------------------------------------------------------
+ * Measurement helper (no reference counterpart): copies `bytes` between two
+ * fresh device buffers with a 16-byte nontemporal streaming kernel, reps + 1
+ * times, and reports the best time and the read + write rate in GB/s. The
+ * benchmark prints it beside the published HBM peak, and its known traffic
+ * scales the FETCH_SIZE / WRITE_SIZE counters of profiles.
+ * ------------------------------------------------------------------------ */
+int ion_hbm_copy_probe(uint64_t bytes, uint32_t reps, double* best_ms, double* gb_per_s);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* ion_last_error(void);
 

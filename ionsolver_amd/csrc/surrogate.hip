@@ -30,7 +30,68 @@ __device__ __forceinline__ float bf2f(__bf16 x) { return (float)x; }
 // operands with leading dimensions lda/ldb (elements), K % 32 == 0.
 // 64 x 64 tile per 256-thread block, 4 waves of 32 x 32 (2 x 2 MFMA tiles).
 // ---------------------------------------------------------------------------
+// Lagrange basis of the three sample positions t_k = ((k + 1/2)/3) b - 1/2
+// (spacing h = b/3) at offset u.
+__device__ __forceinline__ void lagrange3(float u, float b, float L[3]) {
+    const float h = b / 3.0f, t0 = 0.5f * h - 0.5f;
+    const float s = (u - t0) / h;  // sample coordinate: nodes at 0, 1, 2
+    L[0] = 0.5f * (s - 1.0f) * (s - 2.0f);
+    L[1] = -s * (s - 2.0f);
+    L[2] = 0.5f * s * (s - 1.0f);
+}
+
+// Far field of output column (component c, cell offset ox, oy, oz) of LOD
+// block m: tri-quadratic interpolation of the block's 27 samples.
+__device__ __forceinline__ float far_at(const GemmOut& o, uint32_t m, uint32_t c, uint32_t ox, uint32_t oy,
+                                        uint32_t oz) {
+    float Lx[3], Ly[3], Lz[3];
+    lagrange3((float)ox, (float)o.bx, Lx);
+    lagrange3((float)oy, (float)o.by, Ly);
+    lagrange3((float)oz, (float)o.bz, Lz);
+    const float* f = o.far + (size_t)m * FAR_PTS * 6 + c;
+    float acc = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) acc = __builtin_fmaf(Lz[i] * Ly[j] * Lx[k], f[(i * 9 + j * 3 + k) * 6], acc);
+    return acc;
+}
+
 constexpr int BM = 64, BN = 64, BK = 32, LDS_PAD = 8;
+
+// Epilogue of one output element C[m][n] = v (alpha applied here).
+template <int EPI>
+__device__ __forceinline__ void epilogue(const GemmOut& out, int M, int N, int m, int n, float acc) {
+    if (m >= M || n >= N) return;
+    const float v = out.alpha * acc;
+    if (EPI == EPI_F32) {
+        float* p = out.f32 + (size_t)m * out.ldc + n;
+        *p = out.accumulate ? *p + v : v;
+    } else if (EPI == EPI_BF16) {
+        out.bf16[(size_t)m * out.ldc + n] = f2bf(v);
+    } else {  // EPI_HEAD: m = LOD block, n = component * cells_per_block + offset
+        const uint32_t c = (uint32_t)n / out.cpb, o = (uint32_t)n - c * out.cpb;
+        const uint32_t ox = o % out.bx, oy = (o / out.bx) % out.by, oz = o / (out.bx * out.by);
+        const uint32_t X = (uint32_t)m % out.nd, Y = ((uint32_t)m / out.nd) % out.nd, Z = (uint32_t)m / (out.nd * out.nd);
+        const uint64_t cell =
+            (uint64_t)(X * out.bx + ox) + ((uint64_t)(Y * out.by + oy) + (uint64_t)(Z * out.bz + oz) * out.ny) * out.nx;
+        if (out.fine != nullptr) {  // training: network-unit output (net + far field)
+            const float far = far_at(out, (uint32_t)m, c, ox, oy, oz);
+            out.fine[(size_t)c * out.ncell + cell] = acc + far * (out.gain / *out.in_scale);
+        } else {
+            if ((out.flags[cell] & 0x1F) == 0x01) return;
+            const float far = far_at(out, (uint32_t)m, c, ox, oy, oz);
+            const float s = c < 3 ? out.scale_e : out.scale_b;
+            const float kf = c < 3 ? out.ke : out.kmu;
+            const float* stat = c < 3 ? out.e_stat : out.b_stat;
+            float* dyn = c < 3 ? out.e_dyn : out.b_dyn;
+            const size_t idx = (size_t)(c % 3) * out.ncell + cell;
+            dyn[idx] = (stat[idx] + (*out.in_scale) * s * acc) + kf * far;
+        }
+    }
+}
 
 template <int EPI>
 __global__ void __launch_bounds__(256) k_gemm_nt(const __bf16* __restrict__ A, const __bf16* __restrict__ B, int M,
@@ -72,35 +133,80 @@ __global__ void __launch_bounds__(256) k_gemm_nt(const __bf16* __restrict__ A, c
 #pragma unroll
         for (int j = 0; j < 2; j++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
-                const int n = n0 + wn + j * 16 + (lane & 15);
-                if (m >= M || n >= N) continue;
-                const float v = out.alpha * acc[i][j][r];
-                if (EPI == EPI_F32) {
-                    float* p = out.f32 + (size_t)m * out.ldc + n;
-                    *p = out.accumulate ? *p + v : v;
-                } else if (EPI == EPI_BF16) {
-                    out.bf16[(size_t)m * out.ldc + n] = f2bf(v);
-                } else {  // EPI_HEAD: m = LOD block, n = component * cells_per_block + offset
-                    const uint32_t c = (uint32_t)n / out.cpb, o = (uint32_t)n - c * out.cpb;
-                    const uint32_t ox = o % out.bx, oy = (o / out.bx) % out.by, oz = o / (out.bx * out.by);
-                    const uint32_t X = (uint32_t)m % out.nd, Y = ((uint32_t)m / out.nd) % out.nd,
-                                   Z = (uint32_t)m / (out.nd * out.nd);
-                    const uint64_t cell = (uint64_t)(X * out.bx + ox) +
-                                          ((uint64_t)(Y * out.by + oy) + (uint64_t)(Z * out.bz + oz) * out.ny) * out.nx;
-                    if (out.fine != nullptr) {  // training: raw normalised output
-                        out.fine[(size_t)c * out.ncell + cell] = acc[i][j][r];
-                    } else {
-                        if ((out.flags[cell] & 0x1F) == 0x01) continue;
-                        const float s = c < 3 ? out.scale_e : out.scale_b;
-                        const float* stat = c < 3 ? out.e_stat : out.b_stat;
-                        float* dyn = c < 3 ? out.e_dyn : out.b_dyn;
-                        const size_t idx = (size_t)(c % 3) * out.ncell + cell;
-                        dyn[idx] = stat[idx] + (*out.in_scale) * s * acc[i][j][r];
-                    }
-                }
-            }
+            for (int r = 0; r < 4; r++)
+                epilogue<EPI>(out, M, N, m0 + wm + i * 16 + (lane >> 4) * 4 + r, n0 + wn + j * 16 + (lane & 15),
+                              acc[i][j][r]);
+}
+
+// 128 x 128 tile per 256-thread block, 4 waves of 64 x 64 (4 x 4 MFMA tiles);
+// the next K slice is fetched into registers while the current one is
+// multiplied out of LDS.
+constexpr int BM2 = 128, BN2 = 128;
+
+template <int EPI>
+__global__ void __launch_bounds__(256) k_gemm_nt_128(const __bf16* __restrict__ A, const __bf16* __restrict__ B, int M,
+                                                     int N, int K, int lda, int ldb, GemmOut out) {
+    __shared__ __bf16 As[BM2][BK + LDS_PAD];
+    __shared__ __bf16 Bs[BN2][BK + LDS_PAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * BM2, n0 = blockIdx.x * BN2;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+    const int lr = tid >> 2, lk = (tid & 3) * 8;  // loader: rows lr and lr + 64, k offset
+    bf16x8 va[2], vb[2];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int ra = m0 + lr + 64 * h, rb = n0 + lr + 64 * h;
+            va[h] = (bf16x8){};
+            vb[h] = (bf16x8){};
+            if (ra < M) va[h] = *(const bf16x8*)(A + (size_t)ra * lda + k0 + lk);
+            if (rb < N) vb[h] = *(const bf16x8*)(B + (size_t)rb * ldb + k0 + lk);
+        }
+    };
+    fetch(0);
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            *(bf16x8*)&As[lr + 64 * h][lk] = va[h];
+            *(bf16x8*)&Bs[lr + 64 * h][lk] = vb[h];
+        }
+        __syncthreads();
+        if (k0 + BK < K) fetch(k0 + BK);
+        bf16x8 fa[4], fb[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) fa[i] = *(const bf16x8*)&As[wm + i * 16 + fr][fk];
+#pragma unroll
+        for (int j = 0; j < 4; j++) fb[j] = *(const bf16x8*)&Bs[wn + j * 16 + fr][fk];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    }
+    // Epilogue through LDS, one 16-row slice of each wave's 64 x 64 tile at a
+    // time: the slice is written in the MFMA layout and read back row-major,
+    // so that consecutive lanes handle consecutive columns (coalesced stores)
+    // and the per-output epilogue runs in a loop instead of 64 unrolled copies.
+    __shared__ float Cs[4][16][64 + 4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) Cs[wave][(lane >> 4) * 4 + r][j * 16 + (lane & 15)] = acc[i][j][r];
+        __syncthreads();
+#pragma unroll 1
+        for (int t = 0; t < 16; t++)
+            epilogue<EPI>(out, M, N, m0 + wm + i * 16 + t, n0 + wn + lane, Cs[wave][t][lane]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -181,55 +287,6 @@ __global__ void __launch_bounds__(FAR_THREADS) k_far_samples(FarArgs a) {
         for (uint32_t c = 0; c < FAR_CHUNKS; c++) sum += part[c][tid];
         const uint32_t r = tid / 18, k = (tid / 6) % 3, comp = tid % 6;
         a.far[((size_t)blk * FAR_PTS + k + 3 * r) * 6 + comp] = sum;
-    }
-}
-
-// Lagrange basis of the three sample positions t_k = ((k + 1/2)/3) b - 1/2
-// (spacing h = b/3) at offset u.
-__device__ __forceinline__ void lagrange3(float u, float b, float L[3]) {
-    const float h = b / 3.0f, t0 = 0.5f * h - 0.5f;
-    const float s = (u - t0) / h;  // sample coordinate: nodes at 0, 1, 2
-    L[0] = 0.5f * (s - 1.0f) * (s - 2.0f);
-    L[1] = -s * (s - 2.0f);
-    L[2] = 0.5f * s * (s - 1.0f);
-}
-
-__global__ void __launch_bounds__(256) k_far_interp(InterpArgs a) {
-    const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= a.ncell) return;
-    const bool train = a.y != nullptr;
-    if (!train && (a.flags[n] & 0x1F) == 0x01) return;
-    const uint32_t z = (uint32_t)(n / ((uint64_t)a.nx * a.ny)), r = (uint32_t)(n % ((uint64_t)a.nx * a.ny));
-    const uint32_t y = r / a.nx, x = r % a.nx;
-    const uint32_t blk = x / a.lbx + (y / a.lby + (z / a.lbz) * a.nd) * a.nd;
-    float Lx[3], Ly[3], Lz[3];
-    lagrange3((float)(x % a.lbx), (float)a.lbx, Lx);
-    lagrange3((float)(y % a.lby), (float)a.lby, Ly);
-    lagrange3((float)(z % a.lbz), (float)a.lbz, Lz);
-    const float* f = a.far + (size_t)blk * FAR_PTS * 6;
-    float acc[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const float w = Lz[i] * Ly[j] * Lx[k];
-                const float* fp = f + (i * 9 + j * 3 + k) * 6;
-#pragma unroll
-                for (int c = 0; c < 6; c++) acc[c] = __builtin_fmaf(w, fp[c], acc[c]);
-            }
-    const uint64_t N = a.ncell;
-    if (train) {
-        const float tg = a.gain / *a.in_scale;
-#pragma unroll
-        for (int c = 0; c < 6; c++) a.y[(size_t)c * N + n] += acc[c] * tg;
-    } else {
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            a.e_dyn[(size_t)c * N + n] += a.ke * acc[c];
-            a.b_dyn[(size_t)c * N + n] += a.kmu * acc[3 + c];
-        }
     }
 }
 
@@ -477,6 +534,13 @@ static inline dim3 grid_for(size_t n) {
 hipError_t gemm_nt(int epi, const __bf16* A, const __bf16* B, int M, int N, int K, int lda, int ldb, const GemmOut& o,
                    hipStream_t s) {
     if (K % BK != 0 || M <= 0 || N <= 0) return hipErrorInvalidValue;
+    if (M >= BM2 && N >= BN2) {
+        const dim3 g((N + BN2 - 1) / BN2, (M + BM2 - 1) / BM2);
+        if (epi == EPI_F32) hipLaunchKernelGGL(k_gemm_nt_128<EPI_F32>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
+        else if (epi == EPI_BF16) hipLaunchKernelGGL(k_gemm_nt_128<EPI_BF16>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
+        else hipLaunchKernelGGL(k_gemm_nt_128<EPI_HEAD>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
+        return hipGetLastError();
+    }
     const dim3 g((N + BN - 1) / BN, (M + BM - 1) / BM);
     if (epi == EPI_F32) hipLaunchKernelGGL(k_gemm_nt<EPI_F32>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
     else if (epi == EPI_BF16) hipLaunchKernelGGL(k_gemm_nt<EPI_BF16>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
@@ -485,10 +549,6 @@ hipError_t gemm_nt(int epi, const __bf16* A, const __bf16* B, int M, int N, int 
 }
 hipError_t far_samples(const FarArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_far_samples, dim3(a.nblk), dim3(FAR_THREADS), 0, s, a);
-    return hipGetLastError();
-}
-hipError_t far_interp(const InterpArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_far_interp, dim3((uint32_t)((a.ncell + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 hipError_t input(const float4* qu, uint32_t n_top, uint32_t lo, float* scale, __bf16* x, hipStream_t s) {

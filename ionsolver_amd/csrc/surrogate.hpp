@@ -26,6 +26,8 @@ struct GemmOut {
     const uint8_t* flags;
     const float* in_scale;    // input normalisation (device scalar)
     float scale_e, scale_b;   // KE / gain, KMU / gain (gain: network output units per raw field unit)
+    float ke, kmu, gain;
+    const float* far;         // far-field samples [block][27][6], interpolated in the epilogue
     uint64_t ncell;
     uint32_t nx, ny, nz, nd, bx, by, bz, cpb;
 };
@@ -48,23 +50,7 @@ struct FarArgs {
     uint32_t nd, D, nblk;
     uint32_t lbx, lby, lbz;    // cells per block side
 };
-// Tri-quadratic interpolation of the far samples onto the cells, added to
-// E_dyn / B_dyn (k = KE, KMU; fluid cells) or, when y != null, to the
-// training output y [6][N] in network units (times gain / s).
-struct InterpArgs {
-    const float* far;
-    float* e_dyn;
-    float* b_dyn;
-    const uint8_t* flags;
-    float* y;
-    const float* in_scale;
-    float ke, kmu, gain;
-    uint64_t ncell;
-    uint32_t nx, ny, nd, lbx, lby, lbz;
-};
-
 hipError_t far_samples(const FarArgs& a, hipStream_t s);
-hipError_t far_interp(const InterpArgs& a, hipStream_t s);
 hipError_t gemm_nt(int epi, const __bf16* A, const __bf16* B, int M, int N, int K, int lda, int ldb, const GemmOut& o,
                    hipStream_t s);
 hipError_t input(const float4* qu, uint32_t n_top, uint32_t lo, float* scale, __bf16* x, hipStream_t s);

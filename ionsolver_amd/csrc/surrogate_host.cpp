@@ -262,6 +262,10 @@ GemmOut Net::head_out(const Fields& f, float* fine) const {
     o.in_scale = scale_;
     o.scale_e = geo_.ke / gain_;
     o.scale_b = geo_.kmu / gain_;
+    o.ke = geo_.ke;
+    o.kmu = geo_.kmu;
+    o.gain = gain_;
+    o.far = far_;
     o.ncell = ncell_;
     o.nx = geo_.nx;
     o.ny = geo_.ny;
@@ -274,41 +278,23 @@ GemmOut Net::head_out(const Fields& f, float* fine) const {
     return o;
 }
 
-// Far-field samples of every block, interpolated onto the cells: added to
-// E_dyn / B_dyn (y == null) or to the training output y in network units.
-int Net::far_field(const Fields& f, float* y) {
+// Far-field samples of every block (interpolated onto the cells by the
+// head GEMM's epilogue).
+int Net::far_field(const Fields& f) {
     if (f.src_pq == nullptr || f.src_vd == nullptr || f.src_count == nullptr) {
         err_ = "no LOD sources for the far field";
         return -2;
     }
     FarArgs fa{f.src_pq, f.src_vd, f.src_count, far_, nd_, geo_.D, nblk_, geo_.nx / nd_, geo_.ny / nd_, geo_.nz / nd_};
     SCHK(far_samples(fa, stream_));
-    InterpArgs ia{};
-    ia.far = far_;
-    ia.e_dyn = f.e_dyn;
-    ia.b_dyn = f.b_dyn;
-    ia.flags = f.flags;
-    ia.y = y;
-    ia.in_scale = scale_;
-    ia.ke = geo_.ke;
-    ia.kmu = geo_.kmu;
-    ia.gain = gain_;
-    ia.ncell = ncell_;
-    ia.nx = geo_.nx;
-    ia.ny = geo_.ny;
-    ia.nd = nd_;
-    ia.lbx = geo_.nx / nd_;
-    ia.lby = geo_.ny / nd_;
-    ia.lbz = geo_.nz / nd_;
-    SCHK(far_interp(ia, stream_));
     return 0;
 }
 
 int Net::infer(const Fields& f) {
-    int rc = forward_features(f.qu);
-    if (rc) return rc;
+    int rc;
+    if ((rc = forward_features(f.qu)) || (rc = far_field(f))) return rc;
     SCHK(gemm_nt(EPI_HEAD, hcol_, hw_, (int)nblk_, (int)nout_, hk_, hk_, hk_, head_out(f, nullptr), stream_));
-    return far_field(f, nullptr);
+    return 0;
 }
 
 int Net::loss(const Fields& f, double* rel_e, double* rel_b) {
@@ -332,9 +318,8 @@ int Net::loss(const Fields& f, double* rel_e, double* rel_b) {
 
 int Net::residual(const Fields& f, double* rel_e, double* rel_b) {
     int rc;
-    if ((rc = ensure_training_buffers()) || (rc = forward_features(f.qu))) return rc;
+    if ((rc = ensure_training_buffers()) || (rc = forward_features(f.qu)) || (rc = far_field(f))) return rc;
     SCHK(gemm_nt(EPI_HEAD, hcol_, hw_, (int)nblk_, (int)nout_, hk_, hk_, hk_, head_out(f, y_), stream_));
-    if ((rc = far_field(f, y_))) return rc;
     return loss(f, rel_e, rel_b);
 }
 
@@ -399,9 +384,9 @@ int Net::backward() {
 
 int Net::train_step(const Fields& f, bool update, double* rel_e, double* rel_b) {
     int rc;
-    if ((rc = ensure_training_buffers()) || (rc = forward_features(f.qu))) return rc;
+    if ((rc = ensure_training_buffers()) || (rc = forward_features(f.qu)) || (rc = far_field(f))) return rc;
     SCHK(gemm_nt(EPI_HEAD, hcol_, hw_, (int)nblk_, (int)nout_, hk_, hk_, hk_, head_out(f, y_), stream_));
-    if ((rc = far_field(f, y_)) || (rc = loss(f, rel_e, rel_b))) return rc;
+    if ((rc = loss(f, rel_e, rel_b))) return rc;
     SCHK(hipMemcpyAsync(norms_, norm_host_, 2 * sizeof(float), hipMemcpyHostToDevice, stream_));
     HeadGeom hg{ncell_, geo_.nx, geo_.ny, geo_.nz, nd_, geo_.nx / nd_, geo_.ny / nd_, geo_.nz / nd_, cpb_,
                 (uint32_t)nout_pad_, (uint32_t)nblk_pad_};

@@ -81,7 +81,7 @@ class Net {
     int ensure_training_buffers();
     void near_field_head(std::vector<float>& w) const;
     GemmOut head_out(const Fields& f, float* fine) const;
-    int far_field(const Fields& f, float* y);
+    int far_field(const Fields& f);
 
     Geom geo_{};
     Config cfg_{};

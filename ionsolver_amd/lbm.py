@@ -167,6 +167,7 @@ ABI = {
     "ion_surrogate_residual": (_I, [_VP, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "ion_set_field_solver": (_I, [_VP, _I, ctypes.c_uint32, _D]),
     "ion_surrogate_set_learning_rate": (_I, [_VP, ctypes.c_float, ctypes.c_float]),
+    "ion_hbm_copy_probe": (_I, [_U64, ctypes.c_uint32, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "ion_surrogate_get_stats": (_I, [_VP, ctypes.POINTER(_SurStats)]),
     "ion_surrogate_param_count": (_I, [_VP, ctypes.POINTER(_U64)]),
     "ion_surrogate_read_params": (_I, [_VP, _I, _VP, _U64]),
@@ -473,6 +474,13 @@ class Lbm:
     def set_surrogate_params(self, params):
         arr = np.ascontiguousarray(params, np.float32)
         _check(self._lib.ion_surrogate_write_params(self._h, arr.ctypes.data_as(ctypes.c_void_p), arr.size))
+
+
+def hbm_copy_probe(nbytes=1 << 30, reps=10):
+    """Best (ms, GB/s read + write) of a plain 16-byte streaming copy of nbytes."""
+    ms, gbs = ctypes.c_double(), ctypes.c_double()
+    _check(load_library().ion_hbm_copy_probe(int(nbytes), int(reps), ctypes.byref(ms), ctypes.byref(gbs)))
+    return ms.value, gbs.value
 
 
 def device_codec(float_type, direction, data):

@@ -72,15 +72,17 @@ def test_initial_head_is_the_direct_near_field():
     assert rel_l2(b_sur, b_dir) < 1e-2
 
 
-@pytest.mark.parametrize("solid_frac", [0.0, 0.2])
-def test_forward_loss_and_gradients_vs_float64_reference(solid_frac):
-    lbm = make_lbm(16, 3, solid_frac=solid_frac, seed=3)
-    geo = S.Geometry(16, 16, 16, 3, 8)
+@pytest.mark.parametrize("n,solid_frac", [(16, 0.0), (16, 0.2), (32, 0.1)])
+def test_forward_loss_and_gradients_vs_float64_reference(n, solid_frac):
+    """16^3: every GEMM on the 64 x 64 kernel; 32^3: the head and the large
+    input-gradient GEMMs on the 128 x 128 kernel."""
+    lbm = make_lbm(n, 3, solid_frac=solid_frac, seed=3)
+    geo = S.Geometry(n, n, n, 3, 8)
     rng = np.random.default_rng(4)
     params = np.concatenate([rng.standard_normal(r * c) / np.sqrt(27 * 8) for r, c in geo.shapes]).astype(np.float32)
     lbm.set_surrogate_params(params)
     np.testing.assert_array_equal(lbm.surrogate_params(), params)
-    top = random_lod((16, 16, 16), 3, rng)
+    top = random_lod((n, n, n), 3, rng)
     lbm.write_field(Field.QU_LOD, lod_buffer(lbm, top))
     lbm.compute_fields()
     e_dyn = lbm.read_field(Field.E_DYN)
