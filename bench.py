@@ -28,6 +28,20 @@ FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_INTERACTION = 35  # r 3, r^2 5, sqrt 1, cube 2, r/|r|^3 3, e 6, v x pre 9, b 6
 
 
+# HBM bytes per launch of the 256^3 workloads' kernels, measured with
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (profiles/pmc_traffic_256.md), scaled
+# by the factors the same counters give for ion_hbm_copy_probe's 1 GiB copy
+# (FETCH_SIZE x 2, WRITE_SIZE x 1). Kernels compiled since may differ.
+PMC_TRAFFIC = {
+    ("mhd256", "stream_collide"): 6.641e9,
+    ("mhd256-fp16c", "stream_collide"): 3.819e9,
+    ("tgv256", "stream_collide"): 2.910e9,
+    ("mhd256", "update_e_b_dynamic"): 1.553e9,
+    ("mhd256-fp16c", "update_e_b_dynamic"): 1.553e9,
+    ("mhd256-fp16s", "update_e_b_dynamic"): 1.553e9,
+}
+
+
 def bytes_per_cell(cfg):
     """Algorithmic HBM bytes of stream_collide per non-solid cell (each DDF
     read once and written once, fields as the kernel touches them)."""
@@ -201,12 +215,14 @@ def main():
         kernels["field_prep"] = {"ms": round(tm["ms_field_prep"] / k, 4)}
         kernels["clear_qu_lod"] = {"ms": round(tm["ms_other"] / k, 4)}
         roofline = {"bound": "mfma", "achieved": round(eb_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(eb_tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                    "frac": round(eb_tflops / FP32_PEAK_TFLOPS, 4),
+                    "traffic": PMC_TRAFFIC.get((args.workload, "update_e_b_dynamic")),
                     "kernel": "update_e_b_dynamic", "compute": "fp32 VALU"}
     else:
         roofline = {"bound": "hbm", "achieved": kernels["stream_collide"]["GB/s"], "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": kernels["stream_collide"]["frac_hbm"], "traffic": None,
-                    "kernel": "stream_collide"}
+                    "unit": "GB/s", "frac": kernels["stream_collide"]["frac_hbm"],
+                    "traffic": PMC_TRAFFIC.get((args.workload, "stream_collide")), "kernel": "stream_collide"}
+    kernels["stream_collide"]["traffic"] = PMC_TRAFFIC.get((args.workload, "stream_collide"))
     out = {
         "metric": METRIC, "value": round(mlups, 3), "unit": "MLUPs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True,
