@@ -37,7 +37,11 @@ typedef enum { ION_D2Q9 = 0, ION_D3Q15 = 1, ION_D3Q19 = 2, ION_D3Q27 = 3 } ion_v
 /* RelaxationTime, src/lbm/types.rs:55-61 */
 typedef enum { ION_SRT = 0, ION_TRT = 1 } ion_relaxation_time;
 /* FloatType, src/lbm/types.rs:75-83 */
-typedef enum { ION_FP16S = 0, ION_FP16C = 1, ION_FP32 = 2 } ion_float_type;
+/* ION_FP64 has no reference counterpart: the FP64 instantiation of the same
+ * kernels (double DDFs, double fields, double arithmetic, D3Q19 only), the
+ * error reference of the storage formats. Its field buffers hold doubles
+ * (ion_field_size reports 8-byte elements); it has no file format. */
+typedef enum { ION_FP16S = 0, ION_FP16C = 1, ION_FP32 = 2, ION_FP64 = 3 } ion_float_type;
 /* Propellant, src/lbm/units.rs:214-222 */
 typedef enum { ION_PROP_H = 0, ION_PROP_HE, ION_PROP_NE, ION_PROP_AR, ION_PROP_KR, ION_PROP_XE } ion_propellant;
 

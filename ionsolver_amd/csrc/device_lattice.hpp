@@ -10,8 +10,25 @@
 namespace ion {
 
 // ---------------------------------------------------------------------------
+// Arithmetic type R: float for the product kernels, double for the FP64
+// instantiation (the error reference of the storage formats). Every function
+// below runs the same operation sequence in either type.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float fmaR(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaR(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// "1/3" as written in the forcing terms (0.33333334f) and its FP64 value
+template <class R> __device__ __forceinline__ R third();
+template <> __device__ __forceinline__ float third<float>() { return 0.33333334f; }
+template <> __device__ __forceinline__ double third<double>() { return 1.0 / 3.0; }
+// OpenCL clamp(x, lo, hi) = fmin(fmax(x, lo), hi)
+__device__ __forceinline__ float clampR(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ double clampR(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return clampR(x, lo, hi); }
+
+// ---------------------------------------------------------------------------
 // Velocity sets, reference direction ordering (sim_kernels.cl:336-339,
-// neighbor order :277-286, weights src/lbm/domain.rs:757-768).
+// neighbor order :277-286, weights src/lbm/domain.rs:757-768: 1/d0, 1/ds,
+// 1/de evaluated in the arithmetic type).
 // ---------------------------------------------------------------------------
 template <int Q> struct VS;
 template <> struct VS<9> {
@@ -20,7 +37,7 @@ template <> struct VS<9> {
     static constexpr int cy[9] = {0, 0, 0, 1, -1, 1, -1, -1, 1};
     static constexpr int cz[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     static constexpr int cls[9] = {0, 1, 1, 1, 1, 2, 2, 2, 2};
-    static constexpr float w0 = 1.0f / 2.25f, ws = 1.0f / 9.0f, we = 1.0f / 36.0f;
+    static constexpr double d0 = 2.25, ds = 9.0, de = 36.0;
 };
 template <> struct VS<15> {
     static constexpr int dims = 3;
@@ -28,7 +45,7 @@ template <> struct VS<15> {
     static constexpr int cy[15] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, -1, 1, 1, -1};
     static constexpr int cz[15] = {0, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, 1, -1, 1, -1};
     static constexpr int cls[15] = {0, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2};
-    static constexpr float w0 = 1.0f / 4.5f, ws = 1.0f / 9.0f, we = 1.0f / 72.0f;
+    static constexpr double d0 = 4.5, ds = 9.0, de = 72.0;
 };
 template <> struct VS<19> {
     static constexpr int dims = 3;
@@ -36,19 +53,22 @@ template <> struct VS<19> {
     static constexpr int cy[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1};
     static constexpr int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1};
     static constexpr int cls[19] = {0, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
-    static constexpr float w0 = 1.0f / 3.0f, ws = 1.0f / 18.0f, we = 1.0f / 36.0f;
+    static constexpr double d0 = 3.0, ds = 18.0, de = 36.0;
 };
 
-template <int Q> __device__ __forceinline__ constexpr float weight(int i) {
-    return VS<Q>::cls[i] == 0 ? VS<Q>::w0 : (VS<Q>::cls[i] == 1 ? VS<Q>::ws : VS<Q>::we);
+template <int Q, class R> __device__ __forceinline__ constexpr R weight_of_class(int c) {
+    return c == 0 ? R(1) / R(VS<Q>::d0) : (c == 1 ? R(1) / R(VS<Q>::ds) : R(1) / R(VS<Q>::de));
+}
+template <int Q, class R> __device__ __forceinline__ constexpr R weight(int i) {
+    return weight_of_class<Q, R>(VS<Q>::cls[i]);
 }
 
 // Signed sum c_i . (X, Y, Z) over the non-zero components, left to right,
 // the first term carrying its own sign (e.g. "-ux+uy+uz"). Multiplications by
 // the unit direction components are exact and are not performed.
-template <int Q, int I> __device__ __forceinline__ float proj(float X, float Y, float Z) {
+template <int Q, int I, class R> __device__ __forceinline__ R proj(R X, R Y, R Z) {
     constexpr int a = VS<Q>::cx[I], b = VS<Q>::cy[I], c = VS<Q>::cz[I];
-    float s = 0.0f;
+    R s = R(0);
     bool first = true;
     if (a != 0) { s = a > 0 ? X : -X; first = false; }
     if (b != 0) { s = first ? (b > 0 ? Y : -Y) : (b > 0 ? s + Y : s - Y); first = false; }
@@ -69,44 +89,45 @@ template <int Q> struct Unroll<Q, 0> {
 template <int N, class F> __device__ __forceinline__ void static_for(F&& f) { Unroll<N, N>::run(f); }
 
 // calculate_f_eq, sim_kernels.cl:155-207
-template <int Q> __device__ __forceinline__ void f_eq(float rho, float ux, float uy, float uz, float* feq) {
-    const float c3 = -3.0f * ((ux * ux + uy * uy) + uz * uz);
-    const float rhom1 = rho - 1.0f;
-    ux *= 3.0f;
-    uy *= 3.0f;
-    uz *= 3.0f;
-    feq[0] = VS<Q>::w0 * __builtin_fmaf(rho, 0.5f * c3, rhom1);
-    const float rhos = VS<Q>::ws * rho, rhoe = VS<Q>::we * rho;
-    const float rhom1s = VS<Q>::ws * rhom1, rhom1e = VS<Q>::we * rhom1;
+template <int Q, class R> __device__ __forceinline__ void f_eq(R rho, R ux, R uy, R uz, R* feq) {
+    const R c3 = R(-3) * ((ux * ux + uy * uy) + uz * uz);
+    const R rhom1 = rho - R(1);
+    ux *= R(3);
+    uy *= R(3);
+    uz *= R(3);
+    feq[0] = weight_of_class<Q, R>(0) * fmaR(rho, R(0.5) * c3, rhom1);
+    const R ws = weight_of_class<Q, R>(1), we = weight_of_class<Q, R>(2);
+    const R rhos = ws * rho, rhoe = we * rho;
+    const R rhom1s = ws * rhom1, rhom1e = we * rhom1;
     static_for<(Q - 1) / 2>([&](auto p) {
         constexpr int i = 2 * decltype(p)::value + 1;
-        const float s = proj<Q, i>(ux, uy, uz);
-        const float rw = VS<Q>::cls[i] == 1 ? rhos : rhoe;
-        const float rm = VS<Q>::cls[i] == 1 ? rhom1s : rhom1e;
-        const float sq = __builtin_fmaf(s, s, c3);
-        feq[i] = __builtin_fmaf(rw, __builtin_fmaf(0.5f, sq, s), rm);
-        feq[i + 1] = __builtin_fmaf(rw, __builtin_fmaf(0.5f, sq, -s), rm);
+        const R s = proj<Q, i, R>(ux, uy, uz);
+        const R rw = VS<Q>::cls[i] == 1 ? rhos : rhoe;
+        const R rm = VS<Q>::cls[i] == 1 ? rhom1s : rhom1e;
+        const R sq = fmaR(s, s, c3);
+        feq[i] = fmaR(rw, fmaR(R(0.5), sq, s), rm);
+        feq[i + 1] = fmaR(rw, fmaR(R(0.5), sq, -s), rm);
     });
 }
 
 // calculate_rho_u, sim_kernels.cl:208-233: density summed in index order then
 // +1; each velocity component alternates + and - over the direction pairs
 // with a non-zero component, in index order.
-template <int Q> __device__ __forceinline__ void rho_u(const float* f, float& rho, float& ux, float& uy, float& uz) {
-    float r = f[0];
+template <int Q, class R> __device__ __forceinline__ void rho_u(const R* f, R& rho, R& ux, R& uy, R& uz) {
+    R r = f[0];
     static_for<Q - 1>([&](auto k) { r += f[decltype(k)::value + 1]; });
-    r += 1.0f;
-    float u[3] = {0.0f, 0.0f, 0.0f};
+    r += R(1);
+    R u[3] = {R(0), R(0), R(0)};
     static_for<3>([&](auto kk) {
         constexpr int k = decltype(kk)::value;
-        float acc = 0.0f;
+        R acc = R(0);
         bool first = true;
         static_for<(Q - 1) / 2>([&](auto p) {
             constexpr int i = 2 * decltype(p)::value + 1;
             constexpr int c = k == 0 ? VS<Q>::cx[i] : (k == 1 ? VS<Q>::cy[i] : VS<Q>::cz[i]);
             if (c != 0) {
-                const float a = c > 0 ? f[i] : f[i + 1];
-                const float b = c > 0 ? f[i + 1] : f[i];
+                const R a = c > 0 ? f[i] : f[i + 1];
+                const R b = c > 0 ? f[i + 1] : f[i];
                 acc = first ? a - b : (acc + a) - b;
                 first = false;
             }
@@ -121,32 +142,29 @@ template <int Q> __device__ __forceinline__ void rho_u(const float* f, float& rh
 
 // calculate_forcing_terms (Guo), sim_kernels.cl:366-378. The unit direction
 // components enter through exact sign selections.
-template <int Q> __device__ __forceinline__ void forcing(float ux, float uy, float uz, float fx, float fy, float fz, float* Fin) {
-    float uF;
-    if (VS<Q>::dims == 2) uF = -0.33333334f * __builtin_fmaf(ux, fx, uy * fy);
-    else uF = -0.33333334f * __builtin_fmaf(ux, fx, __builtin_fmaf(uy, fy, uz * fz));
-    Fin[0] = (9.0f * VS<Q>::w0) * uF;
+template <int Q, class R> __device__ __forceinline__ void forcing(R ux, R uy, R uz, R fx, R fy, R fz, R* Fin) {
+    R uF;
+    if (VS<Q>::dims == 2) uF = -third<R>() * fmaR(ux, fx, uy * fy);
+    else uF = -third<R>() * fmaR(ux, fx, fmaR(uy, fy, uz * fz));
+    Fin[0] = (R(9) * weight_of_class<Q, R>(0)) * uF;
     static_for<Q - 1>([&](auto k) {
         constexpr int i = decltype(k)::value + 1;
-        constexpr float wi = weight<Q>(i);
-        Fin[i] = (9.0f * wi) * __builtin_fmaf(proj<Q, i>(fx, fy, fz), proj<Q, i>(ux, uy, uz) + 0.33333334f, uF);
+        const R wi = weight<Q, R>(i);
+        Fin[i] = (R(9) * wi) * fmaR(proj<Q, i, R>(fx, fy, fz), proj<Q, i, R>(ux, uy, uz) + third<R>(), uF);
     });
 }
 
 // calculate_a_eq (D3Q7), sim_kernels.cl:390-396
-__device__ __forceinline__ void a_eq(float Qv, float ux, float uy, float uz, float* qeq) {
-    const float wsT4 = 0.5f * Qv, wsTm1 = 0.125f * (Qv - 1.0f);
-    qeq[0] = __builtin_fmaf(0.25f, Qv, -0.25f);
-    qeq[1] = __builtin_fmaf(wsT4, ux, wsTm1);
-    qeq[2] = __builtin_fmaf(wsT4, -ux, wsTm1);
-    qeq[3] = __builtin_fmaf(wsT4, uy, wsTm1);
-    qeq[4] = __builtin_fmaf(wsT4, -uy, wsTm1);
-    qeq[5] = __builtin_fmaf(wsT4, uz, wsTm1);
-    qeq[6] = __builtin_fmaf(wsT4, -uz, wsTm1);
+template <class R> __device__ __forceinline__ void a_eq(R Qv, R ux, R uy, R uz, R* qeq) {
+    const R wsT4 = R(0.5) * Qv, wsTm1 = R(0.125) * (Qv - R(1));
+    qeq[0] = fmaR(R(0.25), Qv, R(-0.25));
+    qeq[1] = fmaR(wsT4, ux, wsTm1);
+    qeq[2] = fmaR(wsT4, -ux, wsTm1);
+    qeq[3] = fmaR(wsT4, uy, wsTm1);
+    qeq[4] = fmaR(wsT4, -uy, wsTm1);
+    qeq[5] = fmaR(wsT4, uz, wsTm1);
+    qeq[6] = fmaR(wsT4, -uz, wsTm1);
 }
-
-// OpenCL clamp(x, lo, hi) = fmin(fmax(x, lo), hi)
-__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 
 // ---------------------------------------------------------------------------
 // DDF storage formats (load/store macros, src/lbm/domain.rs:773-784)
@@ -179,11 +197,13 @@ __device__ __forceinline__ float fp16c_decode(uint16_t x) {
 
 struct F32 {
     typedef float T;
+    typedef float R;
     static __device__ __forceinline__ float ld(float v) { return v; }
     static __device__ __forceinline__ float st(float x) { return x; }
 };
 struct F16S {  // IEEE binary16 scaled by 2^15: vload_half*3.0517578E-5f / vstore_half_rte(x*32768.0f)
     typedef uint16_t T;
+    typedef float R;
     static __device__ __forceinline__ float ld(uint16_t v) {
         return (float)__builtin_bit_cast(_Float16, v) * 3.0517578E-5f;
     }
@@ -197,8 +217,17 @@ struct F16S {  // IEEE binary16 scaled by 2^15: vload_half*3.0517578E-5f / vstor
 };
 struct F16C {
     typedef uint16_t T;
+    typedef float R;
     static __device__ __forceinline__ float ld(uint16_t v) { return fp16c_decode(v); }
     static __device__ __forceinline__ uint16_t st(float x) { return fp16c_encode(x); }
+};
+
+// FP64 instantiation: double DDFs and double arithmetic throughout
+struct F64 {
+    typedef double T;
+    typedef double R;
+    static __device__ __forceinline__ double ld(double v) { return v; }
+    static __device__ __forceinline__ double st(double x) { return x; }
 };
 
 }  // namespace ion

@@ -73,6 +73,8 @@ extern "C" int ion_save(ion_handle* h, const char* path) {
     ion_config c;
     int rc = ion_get_config(h, &c);
     if (rc) return rc;
+    if (c.float_type == ION_FP64)
+        return ion_internal_set_error(ION_ERR_UNSUPPORTED, "the file format has no FP64 float type (FILE_LAYOUT.txt)");
     const uint64_t N = (uint64_t)c.n_x * c.n_y * c.n_z;
     Writer w;
     w.raw(kHeader, 16);

@@ -6,7 +6,7 @@
 
 namespace ion {
 
-enum StoreKind { STORE_FP16S = 0, STORE_FP16C = 1, STORE_FP32 = 2 };
+enum StoreKind { STORE_FP16S = 0, STORE_FP16C = 1, STORE_FP32 = 2, STORE_FP64 = 3 };
 
 // Arguments of initialize / stream_collide / update_fields. Buffers are the
 // reference's device buffers (src/lbm/domain.rs:151-211), all SoA:
@@ -39,6 +39,7 @@ struct LbmArgs {
     float c;              // DEF_C
     float kkge, wq;       // DEF_KKGE, DEF_WQ
     float ils;            // 1 / cells per LOD block
+    double ils_d;         // the same quotient in double (FP64 instantiation)
 };
 
 // Arguments of the E/B field update.
@@ -60,6 +61,7 @@ struct FieldArgs {
     uint32_t near;          // near-field block has more than one cell
     float ke, kmu;
     float* raw;             // if set: the raw source sums (E then B, 6 x N) of fluid cells instead of E_dyn / B_dyn
+    uint32_t f64;           // FP64 instantiation: every field and source array holds doubles
 };
 
 // kernels_lbm.hip
@@ -68,20 +70,20 @@ hipError_t launch_stream_collide(int q, int store, bool mhd, bool trt, bool lod,
                                  const LbmArgs& a, hipStream_t s);
 hipError_t launch_update_fields(int q, int store, const LbmArgs& a, hipStream_t s);
 // Level-2 LOD tree: one block per top-level LOD.
-hipError_t launch_lod_reduce(const float4* seg, float4* qu_lod, uint32_t nx, uint32_t ny, uint32_t nz,
+hipError_t launch_lod_reduce(const void* seg, void* qu_lod, bool f64, uint32_t nx, uint32_t ny, uint32_t nz,
                              uint32_t D, hipStream_t s);
-hipError_t launch_fill_f32(float* p, uint64_t n, float v, hipStream_t s);
+// fill n floats (or doubles when f64) with v
+hipError_t launch_fill(void* p, bool f64, uint64_t n, double v, hipStream_t s);
 hipError_t launch_codec(int store, int dir, const void* in, void* out, uint64_t n, hipStream_t s);
 
 // kernels_field.hip
-hipError_t launch_prep_sources(const float4* qu_lod, uint32_t lo, uint32_t hi, uint32_t nx, uint32_t ny,
-                               uint32_t nz, uint32_t D, float4* src_pq, float4* src_vd, uint32_t* count,
-                               hipStream_t s);
+hipError_t launch_prep_sources(const void* qu_lod, bool f64, uint32_t lo, uint32_t hi, uint32_t nx, uint32_t ny,
+                               uint32_t nz, uint32_t D, void* src_pq, void* src_vd, uint32_t* count, hipStream_t s);
 hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s);
 // E_dyn = E_stat + KE raw_E, B_dyn = B_stat + KMU raw_B for fluid cells (the
 // direct kernel's own epilogue, applied to a.raw).
 hipError_t launch_combine_fields(const FieldArgs& a, hipStream_t s);
-hipError_t launch_sum_f64(const float* x, const uint8_t* flags, uint64_t n, double* partial, uint32_t nblocks,
-                          hipStream_t s);
+hipError_t launch_sum_f64(const void* x, bool f64, const uint8_t* flags, uint64_t n, double* partial,
+                          uint32_t nblocks, hipStream_t s);
 
 }  // namespace ion

@@ -18,9 +18,33 @@ namespace ion {
 // order. A source of zero charge adds exactly zero to every cell, so
 // dropping it leaves all fields unchanged.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_prep_sources(const float4* __restrict__ qu, uint32_t lo, uint32_t hi,
-                                                       uint32_t nd, float dsx, float dsy, float dsz,
-                                                       float4* __restrict__ src_pq, float4* __restrict__ src_vd,
+// 4-vectors of the arithmetic type; the 4th component of a source's vd
+// carries its LOD index (float: the bits, double: the value, exact below 2^53).
+template <class R> struct V4F;
+template <> struct V4F<float> {
+    typedef float4 T;
+    static __device__ __forceinline__ float idx(uint32_t d) { return __uint_as_float(d); }
+    static __device__ __forceinline__ uint32_t idx_of(float v) { return __float_as_uint(v); }
+};
+template <> struct V4F<double> {
+    typedef double4 T;
+    static __device__ __forceinline__ double idx(uint32_t d) { return (double)d; }
+    static __device__ __forceinline__ uint32_t idx_of(double v) { return (uint32_t)v; }
+};
+template <class R> __device__ __forceinline__ typename V4F<R>::T mk4f(R x, R y, R z, R w) {
+    typename V4F<R>::T v;
+    v.x = x;
+    v.y = y;
+    v.z = z;
+    v.w = w;
+    return v;
+}
+
+template <class R>
+__global__ void __launch_bounds__(1024) k_prep_sources(const typename V4F<R>::T* __restrict__ qu, uint32_t lo,
+                                                       uint32_t hi, uint32_t nd, R dsx, R dsy, R dsz,
+                                                       typename V4F<R>::T* __restrict__ src_pq,
+                                                       typename V4F<R>::T* __restrict__ src_vd,
                                                        uint32_t* __restrict__ count) {
     __shared__ uint32_t wave_tot[16];
     __shared__ uint32_t base_sh;
@@ -29,11 +53,11 @@ __global__ void __launch_bounds__(1024) k_prep_sources(const float4* __restrict_
     __syncthreads();
     for (uint32_t c0 = lo; c0 < hi; c0 += 1024) {
         const uint32_t d = c0 + tid;
-        float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        typename V4F<R>::T e = mk4f<R>(R(0), R(0), R(0), R(0));
         bool keep = false;
         if (d < hi) {
             e = qu[d];
-            keep = e.x != 0.0f;
+            keep = e.x != R(0);
         }
         const uint64_t mask = __ballot(keep);
         const uint32_t pre = __popcll(mask & ((1ull << lane) - 1ull));
@@ -45,11 +69,11 @@ __global__ void __launch_bounds__(1024) k_prep_sources(const float4* __restrict_
             const uint32_t o = off + pre;
             const uint32_t t = d % (nd * nd);
             // lod_coordinates, sim_kernels.cl:442-447
-            const float lx = (float)(t % nd) * dsx + 0.5f * dsx;
-            const float ly = (float)(t / nd) * dsy + 0.5f * dsy;
-            const float lz = (float)(d / (nd * nd)) * dsz + 0.5f * dsz;
-            src_pq[o] = make_float4(lx, ly, lz, e.x);
-            src_vd[o] = make_float4(e.y, e.z, e.w, __uint_as_float(d));
+            const R lx = R((float)(t % nd)) * dsx + R(0.5) * dsx;
+            const R ly = R((float)(t / nd)) * dsy + R(0.5) * dsy;
+            const R lz = R((float)(d / (nd * nd))) * dsz + R(0.5) * dsz;
+            src_pq[o] = mk4f<R>(lx, ly, lz, e.x);
+            src_vd[o] = mk4f<R>(e.y, e.z, e.w, V4F<R>::idx(d));
         }
         __syncthreads();
         if (tid == 0) {
@@ -67,15 +91,16 @@ constexpr uint32_t FIELD_BLOCK = 256;
 
 // One interaction in the reference's arithmetic: pre = r / (sqrt(r.r))^3
 // with correctly rounded sqrt and division, e += q*pre, b += q*(v x pre).
-__device__ __forceinline__ void interact_exact(float q, float vx, float vy, float vz, float rx, float ry, float rz,
-                                               float& ex, float& ey, float& ez, float& bx, float& by, float& bz) {
-    const float s = sqrtf((rx * rx + ry * ry) + rz * rz);
-    const float cbm = (s * s) * s;
-    const float prx = rx / cbm, pry = ry / cbm, prz = rz / cbm;
+template <class R>
+__device__ __forceinline__ void interact_exact(R q, R vx, R vy, R vz, R rx, R ry, R rz, R& ex, R& ey, R& ez, R& bx,
+                                               R& by, R& bz) {
+    const R s = sqrt((rx * rx + ry * ry) + rz * rz);
+    const R cbm = (s * s) * s;
+    const R prx = rx / cbm, pry = ry / cbm, prz = rz / cbm;
     ex = ex + q * prx;
     ey = ey + q * pry;
     ez = ez + q * prz;
-    const float crx = vy * prz - vz * pry, cry = vz * prx - vx * prz, crz = vx * pry - vy * prx;
+    const R crx = vy * prz - vz * pry, cry = vz * prx - vx * prz, crz = vx * pry - vy * prx;
     bx = bx + q * crx;
     by = by + q * cry;
     bz = bz + q * crz;
@@ -142,7 +167,7 @@ __global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b(FieldArgs a) {
 #pragma unroll
             for (int k = 0; k < CPT; k++) {
                 const float rx = px[k] - pq.x, ry = py[k] - pq.y, rz = pz[k] - pq.z;
-                if (FAST) {
+                if constexpr (FAST) {
                     const float r2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
                     const float inv = __builtin_amdgcn_rsqf(r2);
                     float t = pq.w * (inv * inv * inv);
@@ -186,6 +211,76 @@ __global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b(FieldArgs a) {
         a.b_dyn[N + n] = a.b_stat[N + n] + a.kmu * by[k];
         a.b_dyn[2 * N + n] = a.b_stat[2 * N + n] + a.kmu * bz[k];
     }
+}
+
+// FP64 instantiation of the field update (the error reference of the
+// storage formats): one cell per work-item, the reference's arithmetic in
+// double, sources staged through LDS.
+constexpr uint32_t FIELD_TILE64 = 256;
+
+__global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b_f64(FieldArgs a) {
+    __shared__ double4 s_pq[FIELD_TILE64];
+    __shared__ double4 s_vd[FIELD_TILE64];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t NX = a.nx, NY = a.ny, NZ = a.nz;
+    const uint32_t nd = 1u << a.D;
+    const uint32_t lbx = NX / nd, lby = NY / nd, lbz = NZ / nd;
+    const uint64_t n64 = (uint64_t)blockIdx.x * FIELD_BLOCK + tid;
+    bool act = n64 < a.N;
+    const uint32_t n = act ? (uint32_t)n64 : 0u;
+    if (act) act = (a.flags[n] & 0x1F) != 0x01;
+    const uint32_t z = n / (NX * NY), t = n - z * NX * NY, y = t / NX, x = t - y * NX;
+    const double px = (double)(float)x, py = (double)(float)y, pz = (double)(float)z;
+    const uint32_t own = x / lbx + (y / lby + z / lbz * nd) * nd;
+    double ex = 0.0, ey = 0.0, ez = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
+    const double* q = (const double*)a.q;
+    const double* u = (const double*)a.u;
+    if (a.near && act) {
+        const uint32_t x0 = (x / a.dsx) * a.dsx, y0 = (y / a.dsy) * a.dsy, z0 = (z / a.dsz) * a.dsz;
+        const uint32_t xu = min(x0 + a.dsx, NX), yu = min(y0 + a.dsy, NY), zu = min(z0 + a.dsz, NZ);
+        for (uint32_t xx = x0; xx < xu; xx++)
+            for (uint32_t yy = y0; yy < yu; yy++)
+                for (uint32_t zz = z0; zz < zu; zz++) {
+                    const uint32_t nc = xx + (yy + zz * NY) * NX;
+                    if (nc == n) continue;
+                    const double qc = q[nc];
+                    if (qc == 0.0) continue;
+                    interact_exact<double>(qc, u[nc], u[(uint64_t)nc + a.N], u[(uint64_t)nc + 2 * a.N],
+                                           px - (double)(float)xx, py - (double)(float)yy, pz - (double)(float)zz,
+                                           ex, ey, ez, bx, by, bz);
+                }
+    }
+    const double4* spq = (const double4*)a.src_pq;
+    const double4* svd = (const double4*)a.src_vd;
+    const uint32_t count = *a.src_count;
+    for (uint32_t base = 0; base < count; base += FIELD_TILE64) {
+        const uint32_t m = min(FIELD_TILE64, count - base);
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += FIELD_BLOCK) {
+            s_pq[i] = spq[base + i];
+            s_vd[i] = svd[base + i];
+        }
+        __syncthreads();
+        for (uint32_t s = 0; s < m; s++) {
+            const double4 pq = s_pq[s];
+            const double4 vd = s_vd[s];
+            if ((uint32_t)vd.w == own) continue;
+            interact_exact<double>(pq.w, vd.x, vd.y, vd.z, px - pq.x, py - pq.y, pz - pq.z, ex, ey, ez, bx, by, bz);
+        }
+    }
+    if (!act) return;
+    const uint64_t N = a.N;
+    double* ed = (double*)a.e_dyn;
+    double* bd = (double*)a.b_dyn;
+    const double* es = (const double*)a.e_stat;
+    const double* bs = (const double*)a.b_stat;
+    const double ke = (double)a.ke, kmu = (double)a.kmu;
+    ed[n] = es[n] + ke * ex;
+    ed[N + n] = es[N + n] + ke * ey;
+    ed[2 * N + n] = es[2 * N + n] + ke * ez;
+    bd[n] = bs[n] + kmu * bx;
+    bd[N + n] = bs[N + n] + kmu * by;
+    bd[2 * N + n] = bs[2 * N + n] + kmu * bz;
 }
 
 // FAST field arithmetic on LOD-aligned tiles. A block covers
@@ -291,7 +386,8 @@ static uint32_t pow2_div(uint32_t v, uint32_t cap) {
 
 // Float64 sum of x over fluid cells (flags & 0x1F != 0x01): one partial per
 // block, summed on the host in block order.
-__global__ void __launch_bounds__(256) k_sum_f64(const float* __restrict__ x, const uint8_t* __restrict__ flags,
+template <class T>
+__global__ void __launch_bounds__(256) k_sum_f64(const T* __restrict__ x, const uint8_t* __restrict__ flags,
                                                  uint64_t n, double* __restrict__ partial) {
     __shared__ double sh[256];
     double acc = 0.0;
@@ -306,16 +402,26 @@ __global__ void __launch_bounds__(256) k_sum_f64(const float* __restrict__ x, co
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
-hipError_t launch_prep_sources(const float4* qu_lod, uint32_t lo, uint32_t hi, uint32_t nx, uint32_t ny, uint32_t nz,
-                               uint32_t D, float4* src_pq, float4* src_vd, uint32_t* count, hipStream_t s) {
+hipError_t launch_prep_sources(const void* qu_lod, bool f64, uint32_t lo, uint32_t hi, uint32_t nx, uint32_t ny,
+                               uint32_t nz, uint32_t D, void* src_pq, void* src_vd, uint32_t* count, hipStream_t s) {
     const uint32_t nd = 1u << D;
-    hipLaunchKernelGGL(k_prep_sources, dim3(1), dim3(1024), 0, s, qu_lod, lo, hi, nd, (float)(nx / nd),
-                       (float)(ny / nd), (float)(nz / nd), src_pq, src_vd, count);
+    const float dx = (float)(nx / nd), dy = (float)(ny / nd), dz = (float)(nz / nd);
+    if (f64)
+        hipLaunchKernelGGL(k_prep_sources<double>, dim3(1), dim3(1024), 0, s, (const double4*)qu_lod, lo, hi, nd,
+                           (double)dx, (double)dy, (double)dz, (double4*)src_pq, (double4*)src_vd, count);
+    else
+        hipLaunchKernelGGL(k_prep_sources<float>, dim3(1), dim3(1024), 0, s, (const float4*)qu_lod, lo, hi, nd, dx,
+                           dy, dz, (float4*)src_pq, (float4*)src_vd, count);
     return hipGetLastError();
 }
 
 hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s) {
     constexpr int CPT_EXACT = 1, CPT_FAST = 2;
+    if (a.f64) {
+        hipLaunchKernelGGL(k_update_e_b_f64, dim3((uint32_t)((a.N + FIELD_BLOCK - 1) / FIELD_BLOCK)),
+                           dim3(FIELD_BLOCK), 0, s, a);
+        return hipGetLastError();
+    }
     if (fast && !a.near && a.D > 0) {
         // LOD-aligned tiles: (ROW*TX) | lod block x, TY | lod block y, TZ | lod block z
         const uint32_t nd = 1u << a.D;
@@ -350,9 +456,10 @@ hipError_t launch_combine_fields(const FieldArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_sum_f64(const float* x, const uint8_t* flags, uint64_t n, double* partial, uint32_t nblocks,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(k_sum_f64, dim3(nblocks), dim3(256), 0, s, x, flags, n, partial);
+hipError_t launch_sum_f64(const void* x, bool f64, const uint8_t* flags, uint64_t n, double* partial,
+                          uint32_t nblocks, hipStream_t s) {
+    if (f64) hipLaunchKernelGGL(k_sum_f64<double>, dim3(nblocks), dim3(256), 0, s, (const double*)x, flags, n, partial);
+    else hipLaunchKernelGGL(k_sum_f64<float>, dim3(nblocks), dim3(256), 0, s, (const float*)x, flags, n, partial);
     return hipGetLastError();
 }
 

@@ -45,6 +45,19 @@ template <int Q, int I> __device__ __forceinline__ uint32_t nb_index(const Nbr& 
     return x + y + z;
 }
 
+// 4-vectors of the arithmetic type (LOD sums)
+template <class R> struct V4;
+template <> struct V4<float> { typedef float4 T; };
+template <> struct V4<double> { typedef double4 T; };
+template <class R> __device__ __forceinline__ typename V4<R>::T mk4(R x, R y, R z, R w) {
+    typename V4<R>::T v;
+    v.x = x;
+    v.y = y;
+    v.z = z;
+    v.w = w;
+    return v;
+}
+
 // Element idx of the array starting at `base` through a 32-bit byte offset,
 // which gives the global load/store form with a scalar 64-bit base and a
 // 32-bit vector offset (one VALU shift per distinct index instead of 64-bit
@@ -66,7 +79,7 @@ template <class T> __device__ __forceinline__ const T* at(const T* base, uint32_
 // the slot selection on the scalar unit); only the cell index varies per lane.
 template <int Q, int NQ, class S>
 __device__ __forceinline__ void load_ddfs(const typename S::T* __restrict__ buf, uint64_t N, uint32_t n,
-                                          const Nbr& b, uint32_t odd, float* fh) {
+                                          const Nbr& b, uint32_t odd, typename S::R* fh) {
     fh[0] = S::ld(*at(buf, n));
     static_for<(NQ - 1) / 2>([&](auto p) {
         constexpr int i = 2 * decltype(p)::value + 1;
@@ -93,14 +106,14 @@ __device__ __forceinline__ void load_raw(const typename S::T* __restrict__ buf, 
     });
 }
 template <int NQ, class S>
-__device__ __forceinline__ void decode(const typename S::T* raw, float* fh) {
+__device__ __forceinline__ void decode(const typename S::T* raw, typename S::R* fh) {
 #pragma unroll
     for (int i = 0; i < NQ; i++) fh[i] = S::ld(raw[i]);
 }
 
 template <int Q, int NQ, class S>
 __device__ __forceinline__ void store_ddfs(typename S::T* __restrict__ buf, uint64_t N, uint32_t n, const Nbr& b,
-                                           uint32_t odd, const float* fh) {
+                                           uint32_t odd, const typename S::R* fh) {
     *at(buf, n) = S::st(fh[0]);
     static_for<(NQ - 1) / 2>([&](auto p) {
         constexpr int i = 2 * decltype(p)::value + 1;
@@ -117,6 +130,22 @@ __device__ __forceinline__ void store_ddfs(typename S::T* __restrict__ buf, uint
 // ---------------------------------------------------------------------------
 template <int Q, class S, bool MHD>
 __global__ void __launch_bounds__(256) k_initialize(LbmArgs a) {
+    typedef typename S::R R;
+    typedef typename V4<R>::T V4;
+    (void)sizeof(V4);
+    R* const Rho = (R*)a.rho;
+    R* const U = (R*)a.u;
+    const R* const Fp = (const R*)a.F;
+    R* const Ed = (R*)a.e_dyn;
+    R* const Bd = (R*)a.b_dyn;
+    const R* const Es = (const R*)a.e_stat;
+    const R* const Bs = (const R*)a.b_stat;
+    R* const Qp = (R*)a.q;
+    V4* const Lseg = (V4*)a.lod_seg;
+    const R W = R(a.w), C = R(a.c), KKGE = R(a.kkge), WQ = R(a.wq), ILS = sizeof(R) == 8 ? R(a.ils_d) : R(a.ils);
+    const R FX = R(a.fx), FY = R(a.fy), FZ = R(a.fz);
+    (void)Rho; (void)U; (void)Fp; (void)Ed; (void)Bd; (void)Es; (void)Bs; (void)Qp; (void)Lseg;
+    (void)W; (void)C; (void)KKGE; (void)WQ; (void)ILS; (void)FX; (void)FY; (void)FZ;
     const uint64_t n64 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n64 >= a.N) return;
     const uint32_t n = (uint32_t)n64;
@@ -124,25 +153,25 @@ __global__ void __launch_bounds__(256) k_initialize(LbmArgs a) {
     const Nbr b = neighbours(n, a.nx, a.ny, a.nz);
     const uint8_t bo = a.flags[n] & 0x1F;
     if (bo == 0x01) {
-        a.u[n] = 0.0f;
-        a.u[N + n] = 0.0f;
-        a.u[2 * N + n] = 0.0f;
-        if (MHD) a.q[n] = 0.0f;
+        U[n] = R(0);
+        U[N + n] = R(0);
+        U[2 * N + n] = R(0);
+        if (MHD) Qp[n] = R(0);
     }
-    const float ux = a.u[n], uy = a.u[N + n], uz = a.u[2 * N + n];
-    float fe[Q];
-    f_eq<Q>(a.rho[n], ux, uy, uz, fe);
+    const R ux = U[n], uy = U[N + n], uz = U[2 * N + n];
+    R fe[Q];
+    f_eq<Q>(Rho[n], ux, uy, uz, fe);
     store_ddfs<Q, Q, S>((typename S::T*)a.fi, N, n, b, 1u, fe);
     if (MHD) {
-        float qeq[7];
-        a_eq(a.q[n], ux, uy, uz, qeq);
+        R qeq[7];
+        a_eq(Qp[n], ux, uy, uz, qeq);
         store_ddfs<19, 7, S>((typename S::T*)a.fqi, N, n, b, 1u, qeq);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            a.b_dyn[k * N + n] = a.b_stat[k * N + n];
-            a.e_dyn[k * N + n] = a.e_stat[k * N + n];
+            Bd[k * N + n] = Bs[k * N + n];
+            Ed[k * N + n] = Es[k * N + n];
         }
-        f_eq<Q>(0.0f, ux, uy, uz, fe);
+        f_eq<Q>(R(0), ux, uy, uz, fe);
         store_ddfs<Q, Q, S>((typename S::T*)a.ei, N, n, b, 1u, fe);
     }
 }
@@ -155,7 +184,24 @@ __global__ void __launch_bounds__(256) k_initialize(LbmArgs a) {
 // ---------------------------------------------------------------------------
 template <int Q, class S, bool MHD, bool TRT, bool LOD>
 __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
-    extern __shared__ float4 lds_lod[];
+    typedef typename S::R R;
+    typedef typename V4<R>::T V4;
+    (void)sizeof(V4);
+    R* const Rho = (R*)a.rho;
+    R* const U = (R*)a.u;
+    const R* const Fp = (const R*)a.F;
+    R* const Ed = (R*)a.e_dyn;
+    R* const Bd = (R*)a.b_dyn;
+    const R* const Es = (const R*)a.e_stat;
+    const R* const Bs = (const R*)a.b_stat;
+    R* const Qp = (R*)a.q;
+    V4* const Lseg = (V4*)a.lod_seg;
+    const R W = R(a.w), C = R(a.c), KKGE = R(a.kkge), WQ = R(a.wq), ILS = sizeof(R) == 8 ? R(a.ils_d) : R(a.ils);
+    const R FX = R(a.fx), FY = R(a.fy), FZ = R(a.fz);
+    (void)Rho; (void)U; (void)Fp; (void)Ed; (void)Bd; (void)Es; (void)Bs; (void)Qp; (void)Lseg;
+    (void)W; (void)C; (void)KKGE; (void)WQ; (void)ILS; (void)FX; (void)FY; (void)FZ;
+    extern __shared__ __align__(32) unsigned char lds_raw[];
+    V4* lds_lod = (V4*)lds_raw;
     const uint32_t tid = threadIdx.x;
     const uint64_t n64 = (uint64_t)blockIdx.x * blockDim.x + tid;
     const uint64_t N = a.N;
@@ -164,7 +210,7 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
     uint8_t bo = 0x01;
     if (active) bo = *at(a.flags, n) & 0x1F;
     active = active && bo != 0x01;
-    float4 lod = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    V4 lod = mk4<R>(R(0), R(0), R(0), R(0));
     typedef typename S::T T;
 
     if (active) {
@@ -172,111 +218,111 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
         const Nbr b = neighbours(n, a.nx, a.ny, a.nz);
         // every load of the cell first: fi, ei, fqi, E_dyn, B_dyn
         T rf[Q], re[MHD ? Q : 1], rq[MHD ? 7 : 1];
-        float Bx = 0.0f, By = 0.0f, Bz = 0.0f, Ex = 0.0f, Ey = 0.0f, Ez = 0.0f;
+        R Bx = R(0), By = R(0), Bz = R(0), Ex = R(0), Ey = R(0), Ez = R(0);
         load_raw<Q, Q, S>((const T*)a.fi, N, n, b, a.odd, rf);
         if (MHD) {
             load_raw<Q, Q, S>((const T*)a.ei, N, n, b, a.odd, re);
             load_raw<19, 7, S>((const T*)a.fqi, N, n, b, a.odd, rq);
-            Bx = *at(a.b_dyn, n); By = *at(a.b_dyn + N, n); Bz = *at(a.b_dyn + 2 * N, n);
-            Ex = *at(a.e_dyn, n); Ey = *at(a.e_dyn + N, n); Ez = *at(a.e_dyn + 2 * N, n);
+            Bx = *at(Bd, n); By = *at(Bd + N, n); Bz = *at(Bd + 2 * N, n);
+            Ex = *at(Ed, n); Ey = *at(Ed + N, n); Ez = *at(Ed + 2 * N, n);
         }
-        float fhn[Q];
+        R fhn[Q];
         decode<Q, S>(rf, fhn);
-        float rhon, uxn, uyn, uzn;
+        R rhon, uxn, uyn, uzn;
         if (is_e) {
-            rhon = *at(a.rho, n);
-            uxn = *at(a.u, n);
-            uyn = *at(a.u + N, n);
-            uzn = *at(a.u + 2 * N, n);
+            rhon = *at(Rho, n);
+            uxn = *at(U, n);
+            uyn = *at(U + N, n);
+            uzn = *at(U + 2 * N, n);
         } else {
             rho_u<Q>(fhn, rhon, uxn, uyn, uzn);
         }
-        float fxn = a.fx, fyn = a.fy, fzn = a.fz;
-        float Fin[Q], feq[Q];
-        const float w = a.w;
-        const float c_tau = __builtin_fmaf(w, -0.5f, 1.0f);
+        R fxn = a.fx, fyn = a.fy, fzn = a.fz;
+        R Fin[Q], feq[Q];
+        const R w = W;
+        const R c_tau = fmaR(w, R(-0.5), R(1));
         if (a.ffield) {
-            fxn += *at(a.F, n);
-            fyn += *at(a.F + N, n);
-            fzn += *at(a.F + 2 * N, n);
+            fxn += *at(Fp, n);
+            fyn += *at(Fp + N, n);
+            fzn += *at(Fp + 2 * N, n);
         }
         if (MHD) {
             // electron gas part 1
-            float ehn[Q];
+            R ehn[Q];
             decode<Q, S>(re, ehn);
-            float rhon_e, uxn_e, uyn_e, uzn_e;
+            R rhon_e, uxn_e, uyn_e, uzn_e;
             rho_u<Q>(ehn, rhon_e, uxn_e, uyn_e, uzn_e);
             // charge advection part 1 (D3Q7 subset of D3Q19 neighbours)
-            float qhn[7];
+            R qhn[7];
             decode<7, S>(rq, qhn);
-            float rhon_q = 0.0f;
+            R rhon_q = R(0);
 #pragma unroll
             for (int i = 0; i < 7; i++) rhon_q += qhn[i];
-            rhon_q += 1.0f;
+            rhon_q += R(1);
             // charge advection part 2
-            *at(a.q, n) = rhon_q - rhon_e;
-            float qeq[7];
+            *at(Qp, n) = rhon_q - rhon_e;
+            R qeq[7];
             a_eq(rhon_q, uxn, uyn, uzn, qeq);
-            const float wq = a.wq;
+            const R wq = WQ;
 #pragma unroll
-            for (int i = 0; i < 7; i++) qhn[i] = __builtin_fmaf(1.0f - wq, qhn[i], wq * qeq[i]);
+            for (int i = 0; i < 7; i++) qhn[i] = fmaR(R(1) - wq, qhn[i], wq * qeq[i]);
             store_ddfs<19, 7, S>((T*)a.fqi, a.N_store, n, b, a.odd, qhn);
             // electron gas part 2: F = -rho_e (E + u_e x B), Guo forcing, SRT
-            const float crx = uyn_e * Bz - uzn_e * By;
-            const float cry = uzn_e * Bx - uxn_e * Bz;
-            const float crz = uxn_e * By - uyn_e * Bx;
-            const float efx = -rhon_e * (Ex + crx), efy = -rhon_e * (Ey + cry), efz = -rhon_e * (Ez + crz);
-            const float rho2_e = 0.5f / (rhon_e * a.kkge);
-            uxn_e = clampf(__builtin_fmaf(efx, rho2_e, uxn_e), -a.c, a.c);
-            uyn_e = clampf(__builtin_fmaf(efy, rho2_e, uyn_e), -a.c, a.c);
-            uzn_e = clampf(__builtin_fmaf(efz, rho2_e, uzn_e), -a.c, a.c);
+            const R crx = uyn_e * Bz - uzn_e * By;
+            const R cry = uzn_e * Bx - uxn_e * Bz;
+            const R crz = uxn_e * By - uyn_e * Bx;
+            const R efx = -rhon_e * (Ex + crx), efy = -rhon_e * (Ey + cry), efz = -rhon_e * (Ez + crz);
+            const R rho2_e = R(0.5) / (rhon_e * KKGE);
+            uxn_e = clampR(fmaR(efx, rho2_e, uxn_e), -C, C);
+            uyn_e = clampR(fmaR(efy, rho2_e, uyn_e), -C, C);
+            uzn_e = clampR(fmaR(efz, rho2_e, uzn_e), -C, C);
             forcing<Q>(uxn_e, uyn_e, uzn_e, efx, efy, efz, Fin);
             f_eq<Q>(rhon_e, uxn_e, uyn_e, uzn_e, feq);
 #pragma unroll
             for (int i = 0; i < Q; i++) {
-                const float fi = Fin[i] * c_tau;
-                ehn[i] = is_e ? feq[i] : __builtin_fmaf(1.0f - w, ehn[i], __builtin_fmaf(w, feq[i], fi));
+                const R fi = Fin[i] * c_tau;
+                ehn[i] = is_e ? feq[i] : fmaR(R(1) - w, ehn[i], fmaR(w, feq[i], fi));
             }
             store_ddfs<Q, Q, S>((T*)a.ei, a.N_store, n, b, a.odd, ehn);
             // EM force on the gas (plain multiply-add, as written)
             fxn += rhon_q * (Ex + uyn * Bz - uzn * By);
             fyn += rhon_q * (Ey + uzn * Bx - uxn * Bz);
             fzn += rhon_q * (Ez + uxn * By - uyn * Bx);
-            if (LOD) lod = make_float4(rhon_q - rhon_e, uxn * a.ils, uyn * a.ils, uzn * a.ils);
+            if (LOD) lod = mk4<R>(rhon_q - rhon_e, uxn * ILS, uyn * ILS, uzn * ILS);
         }
         if (a.vforce) {
-            const float rho2 = 0.5f / rhon;
-            uxn = clampf(__builtin_fmaf(fxn, rho2, uxn), -a.c, a.c);
-            uyn = clampf(__builtin_fmaf(fyn, rho2, uyn), -a.c, a.c);
-            uzn = clampf(__builtin_fmaf(fzn, rho2, uzn), -a.c, a.c);
+            const R rho2 = R(0.5) / rhon;
+            uxn = clampR(fmaR(fxn, rho2, uxn), -C, C);
+            uyn = clampR(fmaR(fyn, rho2, uyn), -C, C);
+            uzn = clampR(fmaR(fzn, rho2, uzn), -C, C);
             forcing<Q>(uxn, uyn, uzn, fxn, fyn, fzn, Fin);
         } else {
-            uxn = clampf(uxn, -a.c, a.c);
-            uyn = clampf(uyn, -a.c, a.c);
-            uzn = clampf(uzn, -a.c, a.c);
+            uxn = clampR(uxn, -C, C);
+            uyn = clampR(uyn, -C, C);
+            uzn = clampR(uzn, -C, C);
 #pragma unroll
-            for (int i = 0; i < Q; i++) Fin[i] = 0.0f;
+            for (int i = 0; i < Q; i++) Fin[i] = R(0);
         }
         if (a.upd && !is_e) {
-            *at(a.rho, n) = rhon;
-            *at(a.u, n) = uxn;
-            *at(a.u + N, n) = uyn;
-            *at(a.u + 2 * N, n) = uzn;
+            *at(Rho, n) = rhon;
+            *at(U, n) = uxn;
+            *at(U + N, n) = uyn;
+            *at(U + 2 * N, n) = uzn;
         }
         f_eq<Q>(rhon, uxn, uyn, uzn, feq);
         if (!TRT) {
             const bool vf = a.vforce != 0;
 #pragma unroll
             for (int i = 0; i < Q; i++) {
-                const float fi = vf ? Fin[i] * c_tau : Fin[i];
-                fhn[i] = is_e ? feq[i] : __builtin_fmaf(1.0f - w, fhn[i], __builtin_fmaf(w, feq[i], fi));
+                const R fi = vf ? Fin[i] * c_tau : Fin[i];
+                fhn[i] = is_e ? feq[i] : fmaR(R(1) - w, fhn[i], fmaR(w, feq[i], fi));
             }
         } else {
-            const float wp = w;
-            const float wm = 1.0f / (0.1875f / (1.0f / w - 0.5f) + 0.5f);
+            const R wp = w;
+            const R wm = R(1) / (R(0.1875) / (R(1) / w - R(0.5)) + R(0.5));
             if (a.vforce) {
-                const float c_taup = __builtin_fmaf(wp, -0.25f, 0.5f), c_taum = __builtin_fmaf(wm, -0.25f, 0.5f);
-                float Fib[Q];
+                const R c_taup = fmaR(wp, R(-0.25), R(0.5)), c_taum = fmaR(wm, R(-0.25), R(0.5));
+                R Fib[Q];
                 Fib[0] = Fin[0];
 #pragma unroll
                 for (int i = 1; i < Q; i += 2) {
@@ -284,9 +330,9 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
                     Fib[i + 1] = Fin[i];
                 }
 #pragma unroll
-                for (int i = 0; i < Q; i++) Fin[i] = __builtin_fmaf(c_taup, Fin[i] + Fib[i], c_taum * (Fin[i] - Fib[i]));
+                for (int i = 0; i < Q; i++) Fin[i] = fmaR(c_taup, Fin[i] + Fib[i], c_taum * (Fin[i] - Fib[i]));
             }
-            float fhb[Q], feb[Q];
+            R fhb[Q], feb[Q];
             fhb[0] = fhn[0];
             feb[0] = feq[0];
 #pragma unroll
@@ -299,8 +345,8 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
 #pragma unroll
             for (int i = 0; i < Q; i++)
                 fhn[i] = is_e ? feq[i]
-                              : __builtin_fmaf(0.5f * wp, feq[i] - fhn[i] + feb[i] - fhb[i],
-                                               __builtin_fmaf(0.5f * wm, feq[i] - feb[i] - fhn[i] + fhb[i], fhn[i] + Fin[i]));
+                              : fmaR(R(0.5) * wp, feq[i] - fhn[i] + feb[i] - fhb[i],
+                                               fmaR(R(0.5) * wm, feq[i] - feb[i] - fhn[i] + fhb[i], fhn[i] + Fin[i]));
         }
         store_ddfs<Q, Q, S>((T*)a.fi, a.N_store, n, b, a.odd, fhn);
     }
@@ -315,13 +361,13 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
         while (P < bx) P <<= 1;
         for (uint32_t s = P >> 1; s >= 1; s >>= 1) {
             if (i < s) {
-                const float4 v = lds_lod[tid];
-                const float4 o = (i + s < bx) ? lds_lod[tid + s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                lds_lod[tid] = make_float4(v.x + o.x, v.y + o.y, v.z + o.z, v.w + o.w);
+                const V4 v = lds_lod[tid];
+                const V4 o = (i + s < bx) ? lds_lod[tid + s] : mk4<R>(R(0), R(0), R(0), R(0));
+                lds_lod[tid] = mk4<R>(v.x + o.x, v.y + o.y, v.z + o.z, v.w + o.w);
             }
             __syncthreads();
         }
-        if (i == 0 && n64 < N) a.lod_seg[n64 / bx] = lds_lod[tid];
+        if (i == 0 && n64 < N) Lseg[n64 / bx] = lds_lod[tid];
     }
 }
 
@@ -330,20 +376,36 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
 // ---------------------------------------------------------------------------
 template <int Q, class S>
 __global__ void __launch_bounds__(256) k_update_fields(LbmArgs a) {
+    typedef typename S::R R;
+    typedef typename V4<R>::T V4;
+    (void)sizeof(V4);
+    R* const Rho = (R*)a.rho;
+    R* const U = (R*)a.u;
+    const R* const Fp = (const R*)a.F;
+    R* const Ed = (R*)a.e_dyn;
+    R* const Bd = (R*)a.b_dyn;
+    const R* const Es = (const R*)a.e_stat;
+    const R* const Bs = (const R*)a.b_stat;
+    R* const Qp = (R*)a.q;
+    V4* const Lseg = (V4*)a.lod_seg;
+    const R W = R(a.w), C = R(a.c), KKGE = R(a.kkge), WQ = R(a.wq), ILS = sizeof(R) == 8 ? R(a.ils_d) : R(a.ils);
+    const R FX = R(a.fx), FY = R(a.fy), FZ = R(a.fz);
+    (void)Rho; (void)U; (void)Fp; (void)Ed; (void)Bd; (void)Es; (void)Bs; (void)Qp; (void)Lseg;
+    (void)W; (void)C; (void)KKGE; (void)WQ; (void)ILS; (void)FX; (void)FY; (void)FZ;
     const uint64_t n64 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n64 >= a.N) return;
     const uint32_t n = (uint32_t)n64;
     const uint64_t N = a.N;
     if ((a.flags[n] & 0x1F) == 0x01) return;
     const Nbr b = neighbours(n, a.nx, a.ny, a.nz);
-    float fhn[Q];
+    R fhn[Q];
     load_ddfs<Q, Q, S>((const typename S::T*)a.fi, N, n, b, a.odd, fhn);
-    float rhon, uxn, uyn, uzn;
+    R rhon, uxn, uyn, uzn;
     rho_u<Q>(fhn, rhon, uxn, uyn, uzn);
-    a.rho[n] = rhon;
-    a.u[n] = clampf(uxn, -a.c, a.c);
-    a.u[N + n] = clampf(uyn, -a.c, a.c);
-    a.u[2 * N + n] = clampf(uzn, -a.c, a.c);
+    Rho[n] = rhon;
+    U[n] = clampR(uxn, -C, C);
+    U[N + n] = clampR(uyn, -C, C);
+    U[2 * N + n] = clampR(uzn, -C, C);
 }
 
 // ---------------------------------------------------------------------------
@@ -351,15 +413,18 @@ __global__ void __launch_bounds__(256) k_update_fields(LbmArgs a) {
 // (index j = yy + zz*by) by the pairwise tree, and writes QU_lod[L]
 // (top-level LODs at offset 0 for a single domain, sim_kernels.cl:667-671).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_lod_reduce(const float4* __restrict__ seg, float4* __restrict__ qu,
-                                                     uint32_t NY, uint32_t nd, uint32_t by, uint32_t bz,
-                                                     uint32_t P) {
-    extern __shared__ float4 a4[];
+template <class R>
+__global__ void __launch_bounds__(1024) k_lod_reduce(const typename V4<R>::T* __restrict__ seg,
+                                                     typename V4<R>::T* __restrict__ qu, uint32_t NY, uint32_t nd,
+                                                     uint32_t by, uint32_t bz, uint32_t P) {
+    typedef typename V4<R>::T V;
+    extern __shared__ __align__(32) unsigned char lds_raw[];
+    V* a4 = (V*)lds_raw;
     const uint32_t L = blockIdx.x;
     const uint32_t X = L % nd, Y = (L / nd) % nd, Z = L / (nd * nd);
     const uint32_t cnt = by * bz;
     for (uint32_t j = threadIdx.x; j < P; j += blockDim.x) {
-        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        V v = mk4<R>(R(0), R(0), R(0), R(0));
         if (j < cnt) {
             const uint32_t yy = j % by, zz = j / by;
             const uint64_t row = (uint64_t)(Z * bz + zz) * NY + (Y * by + yy);
@@ -370,15 +435,15 @@ __global__ void __launch_bounds__(1024) k_lod_reduce(const float4* __restrict__ 
     __syncthreads();
     for (uint32_t s = P >> 1; s >= 1; s >>= 1) {
         for (uint32_t i = threadIdx.x; i < s; i += blockDim.x) {
-            const float4 v = a4[i], o = a4[i + s];
-            a4[i] = make_float4(v.x + o.x, v.y + o.y, v.z + o.z, v.w + o.w);
+            const V v = a4[i], o = a4[i + s];
+            a4[i] = mk4<R>(v.x + o.x, v.y + o.y, v.z + o.z, v.w + o.w);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) qu[L] = a4[0];
 }
 
-__global__ void k_fill_f32(float* p, uint64_t n, float v) {
+template <class R> __global__ void k_fill(R* p, uint64_t n, R v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
@@ -409,7 +474,7 @@ static hipError_t init_impl(bool mhd, const LbmArgs& a, hipStream_t s) {
 template <int Q, class S>
 static hipError_t sc_impl(bool mhd, bool trt, bool lod, uint32_t block, const LbmArgs& a, hipStream_t s) {
     const uint32_t g = blocks_for(a.N, block);
-    const size_t sh = lod ? (size_t)block * sizeof(float4) : 0;
+    const size_t sh = lod ? (size_t)block * sizeof(typename V4<typename S::R>::T) : 0;
 #define ION_SC(M, T, L) hipLaunchKernelGGL((k_stream_collide<Q, S, M, T, L>), dim3(g), dim3(block), sh, s, a)
     if (mhd) {
         if (lod) { if (trt) ION_SC(true, true, true); else ION_SC(true, false, true); }
@@ -428,6 +493,7 @@ static hipError_t uf_impl(const LbmArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_initialize(int q, int store, bool mhd, const LbmArgs& a, hipStream_t s) {
+    if (store == STORE_FP64) return q == 19 ? init_impl<19, F64>(mhd, a, s) : hipErrorInvalidValue;
     if (q == 19) {
         if (store == STORE_FP32) return init_impl<19, F32>(mhd, a, s);
         if (store == STORE_FP16S) return init_impl<19, F16S>(mhd, a, s);
@@ -446,6 +512,7 @@ hipError_t launch_initialize(int q, int store, bool mhd, const LbmArgs& a, hipSt
 
 hipError_t launch_stream_collide(int q, int store, bool mhd, bool trt, bool lod, uint32_t block, const LbmArgs& a,
                                  hipStream_t s) {
+    if (store == STORE_FP64) return q == 19 ? sc_impl<19, F64>(mhd, trt, lod, block, a, s) : hipErrorInvalidValue;
     if (q == 19) {
         if (store == STORE_FP32) return sc_impl<19, F32>(mhd, trt, lod, block, a, s);
         if (store == STORE_FP16S) return sc_impl<19, F16S>(mhd, trt, lod, block, a, s);
@@ -463,6 +530,7 @@ hipError_t launch_stream_collide(int q, int store, bool mhd, bool trt, bool lod,
 }
 
 hipError_t launch_update_fields(int q, int store, const LbmArgs& a, hipStream_t s) {
+    if (store == STORE_FP64) return q == 19 ? uf_impl<19, F64>(a, s) : hipErrorInvalidValue;
     if (q == 19) {
         if (store == STORE_FP32) return uf_impl<19, F32>(a, s);
         if (store == STORE_FP16S) return uf_impl<19, F16S>(a, s);
@@ -479,23 +547,30 @@ hipError_t launch_update_fields(int q, int store, const LbmArgs& a, hipStream_t 
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_lod_reduce(const float4* seg, float4* qu_lod, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t D,
+hipError_t launch_lod_reduce(const void* seg, void* qu_lod, bool f64, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t D,
                              hipStream_t s) {
     const uint32_t nd = 1u << D;
     const uint32_t by = ny / nd, bz = nz / nd;
     (void)nx;
+    (void)nz;
     uint32_t P = 1;
     while (P < by * bz) P <<= 1;
-    if ((size_t)P * sizeof(float4) > 64 * 1024) return hipErrorInvalidValue;
+    const size_t vs = f64 ? sizeof(double4) : sizeof(float4);
+    if ((size_t)P * vs > 64 * 1024) return hipErrorInvalidValue;
     const uint32_t threads = P < 1024 ? (P < 64 ? 64 : P) : 1024;
-    hipLaunchKernelGGL(k_lod_reduce, dim3(nd * nd * nd), dim3(threads), (size_t)P * sizeof(float4), s, seg, qu_lod,
-                       ny, nd, by, bz, P);
+    if (f64)
+        hipLaunchKernelGGL(k_lod_reduce<double>, dim3(nd * nd * nd), dim3(threads), (size_t)P * vs, s,
+                           (const double4*)seg, (double4*)qu_lod, ny, nd, by, bz, P);
+    else
+        hipLaunchKernelGGL(k_lod_reduce<float>, dim3(nd * nd * nd), dim3(threads), (size_t)P * vs, s,
+                           (const float4*)seg, (float4*)qu_lod, ny, nd, by, bz, P);
     return hipGetLastError();
 }
 
-hipError_t launch_fill_f32(float* p, uint64_t n, float v, hipStream_t s) {
+hipError_t launch_fill(void* p, bool f64, uint64_t n, double v, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+    if (f64) hipLaunchKernelGGL(k_fill<double>, dim3(blocks_for(n, 256)), dim3(256), 0, s, (double*)p, n, v);
+    else hipLaunchKernelGGL(k_fill<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s, (float*)p, n, (float)v);
     return hipGetLastError();
 }
 

@@ -104,7 +104,9 @@ struct ion_handle {
     uint32_t fallback_left = 0;
     ion_surrogate_stats sstats{};
 
-    size_t ddf_elem() const { return store == ion::STORE_FP32 ? 4 : 2; }
+    size_t ddf_elem() const { return store == ion::STORE_FP64 ? 8 : (store == ion::STORE_FP32 ? 4 : 2); }
+    bool f64() const { return store == ion::STORE_FP64; }
+    size_t real() const { return f64() ? 8 : 4; }  // element size of the non-DDF float fields
 };
 
 namespace {
@@ -215,6 +217,7 @@ ion::LbmArgs lbm_args(ion_handle* h) {
     // ils = 1.0f/lod_s(D), lod_s = (float)(bx*by*bz), sim_kernels.cl:436-440, 672
     const uint32_t nd = h->nd;
     a.ils = h->lod ? 1.0f / (float)((uint64_t)(h->cfg.n_x / nd) * (h->cfg.n_y / nd) * (h->cfg.n_z / nd)) : 0.0f;
+    a.ils_d = h->lod ? 1.0 / (double)(float)((uint64_t)(h->cfg.n_x / nd) * (h->cfg.n_y / nd) * (h->cfg.n_z / nd)) : 0.0;
     return a;
 }
 
@@ -224,7 +227,7 @@ int prep_sources(ion_handle* h) {
     const uint64_t n_top = (uint64_t)h->nd * h->nd * h->nd;
     const uint32_t hi = (uint32_t)h->n_lod, lo = (uint32_t)(h->n_lod - n_top);
     return timed(h, 2, [&] {
-        return ion::launch_prep_sources((const float4*)h->qu_lod, lo, hi, h->cfg.n_x, h->cfg.n_y, h->cfg.n_z, h->D,
+        return ion::launch_prep_sources(h->qu_lod, h->f64(), lo, hi, h->cfg.n_x, h->cfg.n_y, h->cfg.n_z, h->D,
                                         h->src_pq, h->src_vd, h->src_count, h->stream);
     });
 }
@@ -259,6 +262,7 @@ int field_update(ion_handle* h) {
     f.ke = h->kc.ke;
     f.kmu = h->kc.kmu;
     f.raw = h->field_raw;
+    f.f64 = h->f64() ? 1u : 0u;
     return timed(h, 3, [&] {
         hipError_t e = ion::launch_update_e_b(f, h->field_mode == ION_FIELD_FAST, h->stream);
         if (e == hipSuccess && f.raw != nullptr) e = ion::launch_combine_fields(f, h->stream);
@@ -311,7 +315,7 @@ int field_step(ion_handle* h) {
 int do_step(ion_handle* h) {
     int rc;
     if (h->mhd) {  // clear_qu_lod, sim_kernels.cl:995-1003
-        rc = timed(h, 4, [&] { return ion::launch_fill_f32(h->qu_lod, h->n_lod * 4, 0.0f, h->stream); });
+        rc = timed(h, 4, [&] { return ion::launch_fill(h->qu_lod, h->f64(), h->n_lod * 4, 0.0, h->stream); });
         if (rc) return rc;
     }
     ion::LbmArgs a = lbm_args(h);
@@ -323,7 +327,7 @@ int do_step(ion_handle* h) {
     if (h->mhd) {
         if (h->lod) {
             rc = timed(h, 1, [&] {
-                return ion::launch_lod_reduce(h->lod_seg, (float4*)h->qu_lod, h->cfg.n_x, h->cfg.n_y, h->cfg.n_z, h->D,
+                return ion::launch_lod_reduce(h->lod_seg, h->qu_lod, h->f64(), h->cfg.n_x, h->cfg.n_y, h->cfg.n_z, h->D,
                                               h->stream);
             });
             if (rc) return rc;
@@ -341,7 +345,9 @@ int validate(const ion_config& c) {
         return fail(ION_ERR_UNSUPPORTED, "D3Q27 is not supported: the reference defines no DEF_WC for it (domain.rs:754-756)");
     if (c.velocity_set < ION_D2Q9 || c.velocity_set > ION_D3Q27) return fail(ION_ERR_INVALID, "invalid velocity_set");
     if (c.relaxation_time != ION_SRT && c.relaxation_time != ION_TRT) return fail(ION_ERR_INVALID, "invalid relaxation_time");
-    if (c.float_type < ION_FP16S || c.float_type > ION_FP32) return fail(ION_ERR_INVALID, "invalid float_type");
+    if (c.float_type < ION_FP16S || c.float_type > ION_FP64) return fail(ION_ERR_INVALID, "invalid float_type");
+    if (c.float_type == ION_FP64 && c.velocity_set != ION_D3Q19)
+        return fail(ION_ERR_UNSUPPORTED, "the FP64 instantiation is built for D3Q19 only");
     if (c.n_x == 0 || c.n_y == 0 || c.n_z == 0) return fail(ION_ERR_INVALID, "lattice size must be positive");
     if (c.d_x != 1 || c.d_y != 1 || c.d_z != 1)
         return fail(ION_ERR_UNSUPPORTED, "multi-domain split (d_x, d_y, d_z != 1) is replaced by a single GPU");
@@ -465,27 +471,28 @@ int ion_create(const ion_config* cfg, ion_handle** out) {
     const uint64_t N = h->N;
     const size_t de = h->ddf_elem();
     if ((rc = dalloc(h, &h->fi, N * h->Q * de))) return bail(rc);
-    if ((rc = dalloc(h, (void**)&h->rho, N * 4))) return bail(rc);
-    if ((rc = dalloc(h, (void**)&h->u, 3 * N * 4))) return bail(rc);
+    const size_t rs = h->real();
+    if ((rc = dalloc(h, (void**)&h->rho, N * rs))) return bail(rc);
+    if ((rc = dalloc(h, (void**)&h->u, 3 * N * rs))) return bail(rc);
     if ((rc = dalloc(h, (void**)&h->flags, N))) return bail(rc);
     if ((rc = dalloc(h, (void**)&h->partials, h->n_partials * sizeof(double)))) return bail(rc);
-    if (cfg->ext_force_field && (rc = dalloc(h, (void**)&h->F, 3 * N * 4))) return bail(rc);
+    if (cfg->ext_force_field && (rc = dalloc(h, (void**)&h->F, 3 * N * rs))) return bail(rc);
     if (h->mhd) {
-        if ((rc = dalloc(h, (void**)&h->e_stat, 3 * N * 4))) return bail(rc);
-        if ((rc = dalloc(h, (void**)&h->b_stat, 3 * N * 4))) return bail(rc);
-        if ((rc = dalloc(h, (void**)&h->e_dyn, 3 * N * 4))) return bail(rc);
-        if ((rc = dalloc(h, (void**)&h->b_dyn, 3 * N * 4))) return bail(rc);
+        if ((rc = dalloc(h, (void**)&h->e_stat, 3 * N * rs))) return bail(rc);
+        if ((rc = dalloc(h, (void**)&h->b_stat, 3 * N * rs))) return bail(rc);
+        if ((rc = dalloc(h, (void**)&h->e_dyn, 3 * N * rs))) return bail(rc);
+        if ((rc = dalloc(h, (void**)&h->b_dyn, 3 * N * rs))) return bail(rc);
         if ((rc = dalloc(h, &h->fqi, 7 * N * de))) return bail(rc);
         if ((rc = dalloc(h, &h->ei, N * h->Q * de))) return bail(rc);
-        if ((rc = dalloc(h, (void**)&h->q, N * 4))) return bail(rc);
-        if ((rc = dalloc(h, (void**)&h->qu_lod, h->n_lod * 4 * 4))) return bail(rc);
+        if ((rc = dalloc(h, (void**)&h->q, N * rs))) return bail(rc);
+        if ((rc = dalloc(h, (void**)&h->qu_lod, h->n_lod * 4 * rs))) return bail(rc);
         const uint64_t n_top = (uint64_t)h->nd * h->nd * h->nd;
-        if ((rc = dalloc(h, (void**)&h->src_pq, n_top * sizeof(float4)))) return bail(rc);
-        if ((rc = dalloc(h, (void**)&h->src_vd, n_top * sizeof(float4)))) return bail(rc);
+        if ((rc = dalloc(h, (void**)&h->src_pq, n_top * 4 * rs))) return bail(rc);
+        if ((rc = dalloc(h, (void**)&h->src_vd, n_top * 4 * rs))) return bail(rc);
         if ((rc = dalloc(h, (void**)&h->src_count, sizeof(uint32_t)))) return bail(rc);
-        if (h->lod && (rc = dalloc(h, (void**)&h->lod_seg, (N / h->lod_bx) * sizeof(float4)))) return bail(rc);
+        if (h->lod && (rc = dalloc(h, (void**)&h->lod_seg, (N / h->lod_bx) * 4 * rs))) return bail(rc);
     }
-    e = ion::launch_fill_f32(h->rho, N, 1.0f, h->stream);  // rho buffer starts at 1 (domain.rs:164)
+    e = ion::launch_fill(h->rho, h->f64(), N, 1.0, h->stream);  // rho buffer starts at 1 (domain.rs:164)
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) { fail(ION_ERR_DEVICE, std::string("initial fill: ") + hipGetErrorString(e)); return bail(ION_ERR_DEVICE); }
     *out = h;
@@ -554,23 +561,23 @@ int ion_synchronize(ion_handle* h) {
 
 static void* field_ptr(const ion_handle* h, int field, size_t* bytes, uint32_t* esz) {
     const uint64_t N = h->N;
-    const size_t de = h->ddf_elem();
+    const size_t de = h->ddf_elem(), rs = h->real();
     void* p = nullptr;
     size_t b = 0;
     uint32_t e = 4;
     switch (field) {
         case ION_FIELD_FI: p = h->fi; b = N * h->Q * de; e = (uint32_t)de; break;
-        case ION_FIELD_RHO: p = h->rho; b = N * 4; break;
-        case ION_FIELD_U: p = h->u; b = 3 * N * 4; break;
+        case ION_FIELD_RHO: p = h->rho; b = N * rs; e = (uint32_t)rs; break;
+        case ION_FIELD_U: p = h->u; b = 3 * N * rs; e = (uint32_t)rs; break;
         case ION_FIELD_FLAGS: p = h->flags; b = N; e = 1; break;
-        case ION_FIELD_F: p = h->F; b = h->F ? 3 * N * 4 : 0; break;
-        case ION_FIELD_E_STAT: p = h->e_stat; b = p ? 3 * N * 4 : 0; break;
-        case ION_FIELD_B_STAT: p = h->b_stat; b = p ? 3 * N * 4 : 0; break;
-        case ION_FIELD_E_DYN: p = h->e_dyn; b = p ? 3 * N * 4 : 0; break;
-        case ION_FIELD_B_DYN: p = h->b_dyn; b = p ? 3 * N * 4 : 0; break;
+        case ION_FIELD_F: p = h->F; b = h->F ? 3 * N * rs : 0; e = (uint32_t)rs; break;
+        case ION_FIELD_E_STAT: p = h->e_stat; b = p ? 3 * N * rs : 0; e = (uint32_t)rs; break;
+        case ION_FIELD_B_STAT: p = h->b_stat; b = p ? 3 * N * rs : 0; e = (uint32_t)rs; break;
+        case ION_FIELD_E_DYN: p = h->e_dyn; b = p ? 3 * N * rs : 0; e = (uint32_t)rs; break;
+        case ION_FIELD_B_DYN: p = h->b_dyn; b = p ? 3 * N * rs : 0; e = (uint32_t)rs; break;
         case ION_FIELD_FQI: p = h->fqi; b = p ? 7 * N * de : 0; e = (uint32_t)de; break;
-        case ION_FIELD_Q: p = h->q; b = p ? N * 4 : 0; break;
-        case ION_FIELD_QU_LOD: p = h->qu_lod; b = p ? h->n_lod * 16 : 0; break;
+        case ION_FIELD_Q: p = h->q; b = p ? N * rs : 0; e = (uint32_t)rs; break;
+        case ION_FIELD_QU_LOD: p = h->qu_lod; b = p ? h->n_lod * 4 * rs : 0; e = (uint32_t)rs; break;
         case ION_FIELD_EI: p = h->ei; b = p ? N * h->Q * de : 0; e = (uint32_t)de; break;
         case ION_FIELD_FIELD_SUMS: p = h->field_raw; b = p ? 6 * N * 4 : 0; break;
         default: b = 0;
@@ -620,7 +627,7 @@ int ion_conservation(ion_handle* h, double* mass, double* charge) {
     if (!h) return fail(ION_ERR_INVALID, "null handle");
     std::vector<double> part(h->n_partials);
     auto sum = [&](const float* x, double* out) -> int {
-        hipError_t e = ion::launch_sum_f64(x, h->flags, h->N, h->partials, h->n_partials, h->stream);
+        hipError_t e = ion::launch_sum_f64(x, h->f64(), h->flags, h->N, h->partials, h->n_partials, h->stream);
         if (e != hipSuccess) return fail(ION_ERR_DEVICE, std::string("sum kernel: ") + hipGetErrorString(e));
         HIPCHK(hipStreamSynchronize(h->stream));
         HIPCHK(hipMemcpy(part.data(), h->partials, part.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -672,6 +679,7 @@ void ion_surrogate_config_default(ion_surrogate_config* c) {
 int ion_surrogate_create(ion_handle* h, const ion_surrogate_config* cfg) {
     if (!h || !cfg) return fail(ION_ERR_INVALID, "null argument");
     if (!h->mhd) return fail(ION_ERR_INVALID, "the surrogate needs ext_magneto_hydro");
+    if (h->f64()) return fail(ION_ERR_UNSUPPORTED, "the surrogate runs on FP32 fields, not on the FP64 instantiation");
     if (h->D < 3) return fail(ION_ERR_UNSUPPORTED, "the surrogate needs mhd_lod_depth >= 3");
     const uint32_t sh = 1u << (h->nd & 31u);
     if (h->cfg.n_x / sh > 1 || h->cfg.n_y / sh > 1 || h->cfg.n_z / sh > 1)

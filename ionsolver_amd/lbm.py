@@ -43,9 +43,10 @@ class FloatType(enum.IntEnum):  # types.rs:75-83
     FP16S = 0
     FP16C = 1
     FP32 = 2
+    FP64 = 3   # build-side FP64 instantiation (error reference of the storage formats), D3Q19 only
 
     def size_of(self):
-        return 4 if self == FloatType.FP32 else 2
+        return {FloatType.FP32: 4, FloatType.FP64: 8}.get(self, 2)
 
 
 class Propellant(enum.IntEnum):  # units.rs:214-222
@@ -321,7 +322,7 @@ class LbmConfig:
         return {f: getattr(k, f) for f, _ in k._fields_}
 
 
-_DTYPES = {4: np.float32, 2: np.uint16, 1: np.uint8}
+_DTYPES = {8: np.float64, 4: np.float32, 2: np.uint16, 1: np.uint8}
 
 
 class Lbm:
