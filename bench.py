@@ -145,6 +145,63 @@ def surrogate_leg(lbm, cfg, samples=4, reps=10, channels=16):
             "accuracy_sample": f"{samples} random 8-blob LOD configurations on this lattice (max over samples)"}
 
 
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    d = np.sqrt(np.sum(b * b))
+    return float(np.sqrt(np.sum((a - b) ** 2)) / d) if d > 0 else float(np.sqrt(np.sum(a * a)))
+
+
+def format_errors(n_mhd=128, mhd_steps=3, nan_steps=24, n_tgv=128, tgv_steps=1000):
+    """Relative L2 error of each DDF storage format (FP32 arithmetic) against
+    the FP64 instantiation of the same kernels on the same GPU: the MHD blob
+    workload at n_mhd^3 (EXACT field arithmetic) after mhd_steps steps, the
+    first step with a NaN within nan_steps steps (the electron-gas knife-edge
+    of the reference's initial state), and the Taylor-Green vortex at n_tgv^3
+    after tgv_steps steps."""
+    import ionsolver_amd as I
+    from ionsolver_amd import Field, FieldMode, FloatType, scenarios
+    fmts = [FloatType.FP64, FloatType.FP32, FloatType.FP16S, FloatType.FP16C]
+    mhd, first_nan = {}, {}
+    for ft in fmts:
+        cfg, arrays, _ = scenarios.mhd_blobs(n=n_mhd, float_type=ft, lod_depth=4)
+        lbm = I.Lbm(cfg)
+        scenarios.apply(lbm, arrays)
+        lbm.set_field_mode(FieldMode.EXACT)
+        lbm.initialize()
+        lbm.run(mhd_steps)
+        mhd[ft] = {k: lbm.read_field(f).astype(np.float64) for k, f in
+                   (("rho", Field.RHO), ("u", Field.U), ("E", Field.E_DYN), ("B", Field.B_DYN), ("Q", Field.Q))}
+        first_nan[ft.name] = None
+        for t in range(mhd_steps, nan_steps):
+            if np.isnan(lbm.read_field(Field.Q)).any():
+                first_nan[ft.name] = t
+                break
+            lbm.do_time_step()
+        lbm.close()
+    ref = mhd[FloatType.FP64]
+    table = {}
+    for ft in fmts[1:]:
+        table[ft.name] = {"rho-1": rel_l2(mhd[ft]["rho"] - 1, ref["rho"] - 1), "u": rel_l2(mhd[ft]["u"], ref["u"]),
+                          "E": rel_l2(mhd[ft]["E"], ref["E"]), "B": rel_l2(mhd[ft]["B"], ref["B"]),
+                          "Q": rel_l2(mhd[ft]["Q"], ref["Q"])}
+    tgv = {}
+    for ft in fmts:
+        cfg, arrays = scenarios.fluid_tgv(n=n_tgv, float_type=ft)
+        lbm = I.Lbm(cfg)
+        scenarios.apply(lbm, arrays)
+        lbm.initialize()
+        lbm.run(tgv_steps)
+        tgv[ft] = (lbm.read_field(Field.RHO).astype(np.float64) - 1, lbm.read_field(Field.U).astype(np.float64))
+        lbm.close()
+    tgv_table = {ft.name: {"rho-1": rel_l2(tgv[ft][0], tgv[FloatType.FP64][0]),
+                           "u": rel_l2(tgv[ft][1], tgv[FloatType.FP64][1])} for ft in fmts[1:]}
+    return {"metric": "storage-format rel-L2 vs the FP64 instantiation", "unit": "relative L2",
+            "mhd": {"workload": f"mhd{n_mhd} blobs, D=4, EXACT fields", "steps": mhd_steps, "errors": table,
+                    "first_nan_step": first_nan, "nan_window": nan_steps},
+            "tgv": {"workload": f"tgv{n_tgv}", "steps": tgv_steps, "errors": tgv_table}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +211,8 @@ def main():
     ap.add_argument("--field-mode", default="fast", choices=["exact", "fast"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned field surrogate leg")
+    ap.add_argument("--format-errors", action="store_true",
+                    help="print the storage-format error table against the FP64 instantiation instead")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate", action="store_true",
                     help="first run the HBM copy probe (3 copies of 1 GiB: reads 1 GiB, writes 1 GiB each): "
@@ -167,6 +226,9 @@ def main():
     import ionsolver_amd as I
     from ionsolver_amd import FieldMode, scenarios
 
+    if args.format_errors:
+        print(json.dumps(format_errors()), flush=True)
+        return
     if args.calibrate:
         I.hbm_copy_probe(1 << 30, 2)
     cfg, arrays, meta = workload(args.workload)
