@@ -64,6 +64,10 @@ def workload(name):
         fmt = parts[1] if len(parts) > 1 else "fp32"
         cfg, arrays, blobs = scenarios.mhd_blobs(n=n, float_type=ft[fmt])
         return cfg, arrays, dict(blobs=len(blobs), seed=1234)
+    if name.startswith("thruster"):
+        n = int(name[8:].split("-")[0])
+        cfg, arrays, meta = scenarios.thruster(n=n, log=lambda m: print(m, file=sys.stderr, flush=True))
+        return cfg, arrays, meta
     if name.startswith("tgv"):
         parts = name.split("-")
         n = int(parts[0][3:])
@@ -234,7 +238,14 @@ def main():
     cfg, arrays, meta = workload(args.workload)
     lbm = I.Lbm(cfg)
     scenarios.apply(lbm, arrays)
+    flags = arrays.get("flags")
+    del arrays
     lbm.set_field_mode(FieldMode.FAST if args.field_mode == "fast" else FieldMode.EXACT)
+    thruster = args.workload.startswith("thruster")
+    if thruster:
+        # learned field step, the direct kernel every 50 field steps as judge
+        lbm.surrogate_create(channels=16)
+        lbm.set_field_solver(I.FieldSolver.SURROGATE, judge_every=50, tolerance=0.01)
     lbm.initialize()
     if args.warmup:
         lbm.run(args.warmup)
@@ -247,7 +258,6 @@ def main():
     lbm.set_timing(False)
 
     N = cfg.n_x * cfg.n_y * cfg.n_z
-    flags = arrays.get("flags")
     n_f = int(np.sum((flags & 0x1F) != 1)) if flags is not None else N
     mlups = N * args.steps / dt / 1e6
     k = args.steps
@@ -257,7 +267,13 @@ def main():
                                   "GB/s": round(sc_bytes / sc_ms / 1e6, 1) if sc_ms else None,
                                   "frac_hbm": round(sc_bytes / sc_ms / 1e6 / HBM_PEAK_GBS, 4) if sc_ms else None}}
     roofline = None
-    if cfg.ext_magneto_hydro:
+    if thruster:
+        kernels["field_step"] = {"ms": round(tm["ms_update_e_b"] / k, 4), "solver": "surrogate, judged every 50"}
+        kernels["lod_reduce"] = {"ms": round(tm["ms_lod_reduce"] / k, 4)}
+        roofline = {"bound": "hbm", "achieved": kernels["stream_collide"]["GB/s"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": kernels["stream_collide"]["frac_hbm"], "traffic": None,
+                    "kernel": "stream_collide"}
+    elif cfg.ext_magneto_hydro:
         D = cfg.mhd_lod_depth
         nd = 1 << D
         n_top = nd ** 3
@@ -301,7 +317,11 @@ def main():
                              "note": "best of 10 nontemporal 16-byte copies of 1 GiB (read + write)"}
     kernels["stream_collide"]["frac_of_copy_probe"] = round(kernels["stream_collide"]["GB/s"] / copy_gbs, 4) \
         if kernels["stream_collide"]["GB/s"] else None
-    if cfg.ext_magneto_hydro and not args.no_surrogate and cfg.mhd_lod_depth >= 3:
+    if thruster:
+        out["surrogate"] = {k2: v for k2, v in lbm.surrogate_stats().items()}
+        out["surrogate"]["note"] = ("field steps of the whole run including warm-up; the first is the direct "
+                                    "kernel (judge), whose residual is last_residual_e/b")
+    elif cfg.ext_magneto_hydro and not args.no_surrogate and cfg.mhd_lod_depth >= 3:
         out["surrogate"] = surrogate_leg(lbm, cfg)
     lbm.close()
     if not args.no_cpu_baseline:

@@ -77,18 +77,48 @@ int Net::add_layer(int ca, int cb, int cout, int g, Layer* L) {
     L->cout_pad = pad32(cout);
     L->vox_pad = pad32(L->vox);
     const size_t nw = (size_t)cout * L->kpad;
-    int rc;
-    if ((rc = alloc((void**)&L->w32, nw * 4)) || (rc = alloc((void**)&L->g32, nw * 4)) ||
-        (rc = alloc((void**)&L->m, nw * 4)) || (rc = alloc((void**)&L->v, nw * 4)) ||
-        (rc = alloc((void**)&L->w, nw * 2)) || (rc = alloc((void**)&L->wt, (size_t)L->kpad * L->cout_pad * 2)) ||
+    int rc;  // inference state; the training state comes with the first training step
+    if ((rc = alloc((void**)&L->w32, nw * 4)) || (rc = alloc((void**)&L->w, nw * 2)) ||
+        (rc = alloc((void**)&L->wt, (size_t)L->kpad * L->cout_pad * 2)) ||
         (rc = alloc((void**)&L->col, (size_t)L->vox * L->kpad * 2)) ||
-        (rc = alloc((void**)&L->colT, (size_t)L->kpad * L->vox_pad * 2)) ||
-        (rc = alloc((void**)&L->out, (size_t)L->vox * cout * 2)) ||
-        (rc = alloc((void**)&L->dout32, (size_t)L->vox * cout * 4)) ||
-        (rc = alloc((void**)&L->dout, (size_t)L->vox * L->cout_pad * 2)) ||
-        (rc = alloc((void**)&L->doutT, (size_t)L->cout_pad * L->vox_pad * 2)) ||
-        (rc = alloc((void**)&L->dcol, (size_t)L->vox * L->kpad * 4)))
+        (rc = alloc((void**)&L->out, (size_t)L->vox * cout * 2)))
         return rc;
+    return 0;
+}
+
+// Gradients, Adam moments and backward scratch, allocated at the first
+// training step (an inference-only surrogate never holds them).
+int Net::ensure_training_state() {
+    if (trainable_) return 0;
+    int rc;
+    for (int i = 0; i < NLAYER; i++) {
+        Layer* L = &layers_[i];
+        const size_t nw = (size_t)L->cout * L->kpad;
+        if ((rc = alloc((void**)&L->g32, nw * 4)) || (rc = alloc((void**)&L->m, nw * 4)) ||
+            (rc = alloc((void**)&L->v, nw * 4)) || (rc = alloc((void**)&L->colT, (size_t)L->kpad * L->vox_pad * 2)) ||
+            (rc = alloc((void**)&L->dout32, (size_t)L->vox * L->cout * 4)) ||
+            (rc = alloc((void**)&L->dout, (size_t)L->vox * L->cout_pad * 2)) ||
+            (rc = alloc((void**)&L->doutT, (size_t)L->cout_pad * L->vox_pad * 2)) ||
+            (rc = alloc((void**)&L->dcol, (size_t)L->vox * L->kpad * 4)))
+            return rc;
+    }
+    const int g0 = (int)nd_, C0 = cfg_.c0;
+    const int C[4] = {C0, 2 * C0, 3 * C0, 4 * C0};
+    for (int l = 0; l < 3; l++) {
+        const int g = g0 >> l, h = g / 2;
+        if ((rc = alloc((void**)&dpool_[l], (size_t)h * h * h * C[l] * 4)) ||
+            (rc = alloc((void**)&dup_[l], (size_t)g * g * g * C[l + 1] * 4)) ||
+            (rc = alloc((void**)&de_[l], (size_t)g * g * g * C[l] * 4)))
+            return rc;
+    }
+    if ((rc = alloc((void**)&de_[3], (size_t)(g0 / 8) * (g0 / 8) * (g0 / 8) * C[3] * 4))) return rc;
+    const size_t nh = (size_t)nout_ * hk_;
+    if ((rc = alloc((void**)&hg32_, nh * 4)) || (rc = alloc((void**)&hm_, nh * 4)) || (rc = alloc((void**)&hv_, nh * 4)) ||
+        (rc = alloc((void**)&hcolT_, (size_t)hk_ * nblk_pad_ * 2)) || (rc = alloc((void**)&hdcol_, (size_t)nblk_ * hk_ * 4)) ||
+        (rc = alloc((void**)&dY_, (size_t)nblk_ * nout_pad_ * 2)) ||
+        (rc = alloc((void**)&dYt_, (size_t)nout_pad_ * nblk_pad_ * 2)))
+        return rc;
+    trainable_ = true;
     return 0;
 }
 
@@ -117,16 +147,11 @@ int Net::create(const Geom& geo, const Config& cfg, hipStream_t stream) {
         return rc;
     for (int l = 0; l < 3; l++) {
         const int g = g0 >> l, h = g / 2, c = C[l];
+        // pooled input of encoder level l + 1, upsampled input of decoder level l
         if ((rc = alloc((void**)&pool_[l], (size_t)h * h * h * c * 2)) ||
-            (rc = alloc((void**)&dpool_[l], (size_t)h * h * h * c * 4)))
+            (rc = alloc((void**)&up_[l], (size_t)g * g * g * C[l + 1] * 2)))
             return rc;
-        const int cu = C[l + 1];  // upsampled input of decoder level l
-        if ((rc = alloc((void**)&up_[l], (size_t)g * g * g * (l == 2 ? C[3] : C[l + 1]) * 2)) ||
-            (rc = alloc((void**)&dup_[l], (size_t)g * g * g * cu * 4)))
-            return rc;
-        if ((rc = alloc((void**)&de_[l], (size_t)g * g * g * c * 4))) return rc;
     }
-    if ((rc = alloc((void**)&de_[3], (size_t)(g0 / 8) * (g0 / 8) * (g0 / 8) * C[3] * 4))) return rc;
     // head
     cpb_ = (geo.nx / nd_) * (geo.ny / nd_) * (geo.nz / nd_);
     gain_ = (float)std::pow((double)cpb_, 2.0 / 3.0);
@@ -137,10 +162,8 @@ int Net::create(const Geom& geo, const Config& cfg, hipStream_t stream) {
     hcin_ = C0 + 8;
     hk_ = pad32(27 * hcin_);
     const size_t nh = (size_t)nout_ * hk_;
-    if ((rc = alloc((void**)&hw32_, nh * 4)) || (rc = alloc((void**)&hg32_, nh * 4)) || (rc = alloc((void**)&hm_, nh * 4)) ||
-        (rc = alloc((void**)&hv_, nh * 4)) || (rc = alloc((void**)&hw_, nh * 2)) ||
-        (rc = alloc((void**)&hwt_, (size_t)hk_ * nout_pad_ * 2)) || (rc = alloc((void**)&hcol_, (size_t)nblk_ * hk_ * 2)) ||
-        (rc = alloc((void**)&hcolT_, (size_t)hk_ * nblk_pad_ * 2)) || (rc = alloc((void**)&hdcol_, (size_t)nblk_ * hk_ * 4)))
+    if ((rc = alloc((void**)&hw32_, nh * 4)) || (rc = alloc((void**)&hw_, nh * 2)) ||
+        (rc = alloc((void**)&hwt_, (size_t)hk_ * nout_pad_ * 2)) || (rc = alloc((void**)&hcol_, (size_t)nblk_ * hk_ * 2)))
         return rc;
     ncell_ = geo.ncell;
     n_partials_ = 1024;
@@ -149,13 +172,10 @@ int Net::create(const Geom& geo, const Config& cfg, hipStream_t stream) {
     return init_weights();
 }
 
-int Net::ensure_training_buffers() {
+// Network-unit output at every cell, for residuals and training.
+int Net::ensure_output_buffer() {
     if (y_) return 0;
-    int rc;
-    if ((rc = alloc((void**)&y_, (size_t)6 * ncell_ * 4)) || (rc = alloc((void**)&dY_, (size_t)nblk_ * nout_pad_ * 2)) ||
-        (rc = alloc((void**)&dYt_, (size_t)nout_pad_ * nblk_pad_ * 2)))
-        return rc;
-    return 0;
+    return alloc((void**)&y_, (size_t)6 * ncell_ * 4);
 }
 
 // Exact near field of the 26 neighbouring LOD sources for the cell at offset
@@ -318,7 +338,7 @@ int Net::loss(const Fields& f, double* rel_e, double* rel_b) {
 
 int Net::residual(const Fields& f, double* rel_e, double* rel_b) {
     int rc;
-    if ((rc = ensure_training_buffers()) || (rc = forward_features(f.qu)) || (rc = far_field(f))) return rc;
+    if ((rc = ensure_output_buffer()) || (rc = forward_features(f.qu)) || (rc = far_field(f))) return rc;
     SCHK(gemm_nt(EPI_HEAD, hcol_, hw_, (int)nblk_, (int)nout_, hk_, hk_, hk_, head_out(f, y_), stream_));
     return loss(f, rel_e, rel_b);
 }
@@ -384,7 +404,9 @@ int Net::backward() {
 
 int Net::train_step(const Fields& f, bool update, double* rel_e, double* rel_b) {
     int rc;
-    if ((rc = ensure_training_buffers()) || (rc = forward_features(f.qu)) || (rc = far_field(f))) return rc;
+    if ((rc = ensure_output_buffer()) || (rc = ensure_training_state()) || (rc = forward_features(f.qu)) ||
+        (rc = far_field(f)))
+        return rc;
     SCHK(gemm_nt(EPI_HEAD, hcol_, hw_, (int)nblk_, (int)nout_, hk_, hk_, hk_, head_out(f, y_), stream_));
     if ((rc = loss(f, rel_e, rel_b))) return rc;
     SCHK(hipMemcpyAsync(norms_, norm_host_, 2 * sizeof(float), hipMemcpyHostToDevice, stream_));
@@ -415,6 +437,10 @@ size_t Net::param_count() const {
 // Flat fp32 view of all parameters (or gradients): layers E0..E3, D2, D1, D0,
 // then the head; each [cout][kpad] row-major.
 int Net::copy_params(float* host, bool grads, bool to_device) {
+    if (grads) {
+        const int rc = ensure_training_state();
+        if (rc) return rc;
+    }
     size_t off = 0;
     auto cp = [&](float* dev, size_t n) -> int {
         if (to_device) SCHK(hipMemcpyAsync(dev, host + off, n * 4, hipMemcpyHostToDevice, stream_));

@@ -78,7 +78,8 @@ class Net {
     int forward_features(const float4* qu);
     int backward();
     int loss(const Fields& f, double* rel_e, double* rel_b);
-    int ensure_training_buffers();
+    int ensure_output_buffer();
+    int ensure_training_state();
     void near_field_head(std::vector<float>& w) const;
     GemmOut head_out(const Fields& f, float* fine) const;
     int far_field(const Fields& f);
@@ -113,6 +114,7 @@ class Net {
     float* norms_ = nullptr;
     float norm_host_[2] = {0.0f, 0.0f};
     uint64_t step_ = 0;
+    bool trainable_ = false;
 };
 
 }  // namespace sur

@@ -135,3 +135,91 @@ def random_lod(shape, D, rng, n_blobs=8, sigma=(3.0, 16.0), amp=0.05, vel=0.05, 
         v.append(vel * f / 2)
     out = np.stack([q.ravel(), v[0].ravel(), v[1].ravel(), v[2].ravel()], 1)
     return out.astype(np.float32)
+
+
+def _loop_field(r, z, a, wire):
+    """(B_r, B_z) of a circular current loop of radius a at z = 0 (unit
+    prefactor mu0 I / (2 pi)), closed form with complete elliptic integrals;
+    distances to the wire below `wire` are clamped (finite conductor)."""
+    from scipy.special import ellipe, ellipk
+    d2 = np.maximum((a - r) ** 2 + z * z, wire * wire)
+    s2 = (a + r) ** 2 + z * z
+    m = np.clip(4.0 * a * r / s2, 0.0, 1.0 - 1e-12)
+    K, E = ellipk(m), ellipe(m)
+    s = np.sqrt(s2)
+    bz = (K + (a * a - r * r - z * z) / d2 * E) / s
+    with np.errstate(divide="ignore", invalid="ignore"):
+        br = np.where(r > 0, z / (r * s) * (-K + (a * a + r * r + z * z) / d2 * E), 0.0)
+    return br, bz
+
+
+def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magnet_r=350.0, b_center=0.05,
+             u_inlet=0.02, n_blobs=8, sigma=12.0, amp=0.05, seed=1234, log=None):
+    """Synthetic plasma-thruster workload: a z-aligned quartz tube (solid
+    shell, flags 0x01) of radius tube_r, a ring magnet (current loop of
+    radius magnet_r at mid-height, analytic field scaled to b_center at its
+    centre) as B_stat, an inlet plane (flags 0x02, equilibrium boundary) at
+    z = 0 inside the tube with u = (0, 0, u_inlet), the tube filled with
+    fluid moving at u_inlet and seeded with Gaussian charge blobs. Lengths
+    are given for n = 1024 and scaled with n. Arrays are built one z-layer at
+    a time. Returns (cfg, arrays, meta)."""
+    from .lbm import FloatType as FT
+    ft = FT.FP16C if float_type is None else float_type
+    sc = n / 1024.0
+    rt, th, rm = tube_r * sc, max(wall * sc, 2.0), magnet_r * sc
+    cfg = LbmConfig()
+    cfg.velocity_set = VelocitySet.D3Q19
+    cfg.float_type = ft
+    cfg.n_x = cfg.n_y = cfg.n_z = n
+    cfg.units.set(**MHD_UNITS)
+    cfg.nu = float(np.float32(0.1) / np.float32(3.0))
+    cfg.ext_volume_force = True
+    cfg.ext_magneto_hydro = True
+    cfg.ext_equilibrium_boudaries = True
+    cfg.mhd_lod_depth = lod_depth
+    N = n ** 3
+    c = 0.5 * n
+    yy, xx = np.meshgrid(np.arange(n, dtype=np.float64) - c, np.arange(n, dtype=np.float64) - c, indexing="ij")
+    r = np.sqrt(xx * xx + yy * yy)
+    inside = r < rt
+    shell = (r >= rt) & (r < rt + th)
+    flags = np.zeros(N, np.uint8)
+    u = np.zeros(3 * N, np.float32)
+    q = np.zeros(N, np.float32)
+    b = np.zeros(3 * N, np.float32)
+    br0, bz0 = _loop_field(np.array([0.0]), np.array([0.0]), rm, 2.0)
+    scale = b_center / bz0[0]
+    rng = np.random.default_rng(seed)
+    blobs = []
+    while len(blobs) < n_blobs:
+        p = rng.uniform([c - rt, c - rt, 2 * sigma * sc], [c + rt, c + rt, n - 2 * sigma * sc])
+        if np.hypot(p[0] - c, p[1] - c) > rt - 2 * sigma * sc:
+            continue
+        blobs.append((p[0], p[1], p[2], (1.0 if rng.uniform() < 0.5 else -1.0) * amp * rng.uniform(0.5, 1.0)))
+    s2 = 2.0 * (sigma * sc) ** 2
+    gxy = [a * np.exp(-((xx + c - bx) ** 2 + (yy + c - by) ** 2) / s2) for bx, by, bz, a in blobs]
+    lay = n * n
+    for z in range(n):
+        sl = slice(z * lay, (z + 1) * lay)
+        f = np.zeros(lay, np.uint8)
+        f[shell.ravel()] = 1
+        if z == 0:
+            f[inside.ravel()] = 2
+        flags[sl] = f
+        u[2 * N + z * lay:2 * N + (z + 1) * lay] = np.where(inside, u_inlet, 0.0).ravel()
+        qz = np.zeros((n, n))
+        for g, (bx, by, bz, a) in zip(gxy, blobs):
+            qz += g * np.exp(-((z - bz) ** 2) / s2)
+        q[sl] = np.where(inside, qz, 0.0).ravel()
+        brl, bzl = _loop_field(r, np.full_like(r, z - c), rm, 2.0)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            bxl = np.where(r > 0, brl * xx / r, 0.0)
+            byl = np.where(r > 0, brl * yy / r, 0.0)
+        b[z * lay:(z + 1) * lay] = (scale * bxl).ravel()
+        b[N + z * lay:N + (z + 1) * lay] = (scale * byl).ravel()
+        b[2 * N + z * lay:2 * N + (z + 1) * lay] = (scale * bzl).ravel()
+        if log and z % 128 == 127:
+            log(f"thruster setup: layer {z + 1}/{n}")
+    arrays = dict(flags=flags, u=u, q=q, b_stat=b)
+    meta = dict(tube_r=rt, wall=th, magnet_r=rm, b_center=b_center, u_inlet=u_inlet, blobs=len(blobs), seed=seed)
+    return cfg, arrays, meta

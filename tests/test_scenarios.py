@@ -1,0 +1,46 @@
+"""Host-side workload generators (CPU only)."""
+import numpy as np
+
+from ionsolver_amd import FloatType
+from ionsolver_amd.scenarios import _loop_field, thruster
+
+
+def test_loop_field_closed_form_on_axis():
+    """On the axis of a current loop the field is axial, a^2 / (a^2 + z^2)^1.5
+    in units of mu0 I / 2 (here mu0 I / (2 pi) times pi)."""
+    a = 7.0
+    z = np.array([0.0, 1.0, 3.0, 10.0])
+    br, bz = _loop_field(np.zeros_like(z), z, a, 0.1)
+    np.testing.assert_allclose(br, 0.0, atol=1e-15)
+    np.testing.assert_allclose(bz, np.pi * a * a / (a * a + z * z) ** 1.5, rtol=1e-12)
+
+
+def test_loop_field_is_divergence_free():
+    a, h = 5.0, 1e-4
+    r, z = np.array([1.3, 2.2, 7.5]), np.array([0.4, -1.1, 2.0])
+    # div B = (1/r) d(r B_r)/dr + dB_z/dz
+    brp, _ = _loop_field(r + h, z, a, 0.1)
+    brm, _ = _loop_field(r - h, z, a, 0.1)
+    _, bzp = _loop_field(r, z + h, a, 0.1)
+    _, bzm = _loop_field(r, z - h, a, 0.1)
+    div = ((r + h) * brp - (r - h) * brm) / (2 * h * r) + (bzp - bzm) / (2 * h)
+    _, bz0 = _loop_field(r, z, a, 0.1)
+    assert np.all(np.abs(div) < 1e-6 * np.abs(bz0).max())
+
+
+def test_thruster_geometry():
+    n = 64
+    cfg, arr, meta = thruster(n=n, float_type=FloatType.FP16C, lod_depth=3)
+    f = arr["flags"].reshape(n, n, n)
+    x = np.arange(n) - n / 2
+    r = np.sqrt(x[None, :] ** 2 + x[:, None] ** 2)
+    assert np.all(f[:, (r >= meta["tube_r"]) & (r < meta["tube_r"] + meta["wall"])] == 1)
+    assert np.all(f[0][r < meta["tube_r"]] == 2) and np.all(f[1:][:, r < meta["tube_r"]] == 0)
+    q = arr["q"].reshape(n, n, n)
+    assert np.all(q[:, r >= meta["tube_r"]] == 0) and np.abs(q).max() > 0
+    b = arr["b_stat"].reshape(3, n, n, n)
+    np.testing.assert_allclose(b[:, n // 2, n // 2, n // 2], [0.0, 0.0, meta["b_center"]], atol=1e-7)
+    # mirror symmetry of the loop field about its plane: B_z even, B_x odd in z - n/2
+    np.testing.assert_allclose(b[2, n // 2 + 5], b[2, n // 2 - 5], rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(b[0, n // 2 + 5], -b[0, n // 2 - 5], rtol=1e-5, atol=1e-9)
+    assert cfg.ext_equilibrium_boudaries and cfg.ext_magneto_hydro
