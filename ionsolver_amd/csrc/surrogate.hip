@@ -195,6 +195,26 @@ __global__ void __launch_bounds__(256) k_gemm_nt_128(const __bf16* __restrict__ 
     // so that consecutive lanes handle consecutive columns (coalesced stores)
     // and the per-output epilogue runs in a loop instead of 64 unrolled copies.
     __shared__ float Cs[4][16][64 + 4];
+    // Head with 64-column-aligned components (cells per block a multiple of
+    // 64): a wave's 64 columns share the component, so the 27 far-field
+    // samples of each of its rows are staged in LDS (one broadcast read per
+    // sample) and each lane's 27 interpolation weights stay in registers.
+    __shared__ float Fs[4][16][FAR_PTS];
+    constexpr bool head_lds = EPI == EPI_HEAD;  // launched only with cells per block % 64 == 0
+    float Lx[3], Ly[3], Lz[3];
+    uint32_t hc = 0, hcell0 = 0;
+    float in_s = 1.0f;
+    if (head_lds) {
+        in_s = *out.in_scale;
+        const uint32_t n = (uint32_t)(n0 + wn + lane);
+        hc = (uint32_t)(n0 + wn) / out.cpb;
+        const uint32_t o = n - hc * out.cpb;
+        const uint32_t ox = o % out.bx, oy = (o / out.bx) % out.by, oz = o / (out.bx * out.by);
+        lagrange3((float)ox, (float)out.bx, Lx);
+        lagrange3((float)oy, (float)out.by, Ly);
+        lagrange3((float)oz, (float)out.bz, Lz);
+        hcell0 = ox + (oy + oz * out.ny) * out.nx;  // cell offset of the lane within its block
+    }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         __syncthreads();
@@ -202,10 +222,73 @@ __global__ void __launch_bounds__(256) k_gemm_nt_128(const __bf16* __restrict__ 
         for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int r = 0; r < 4; r++) Cs[wave][(lane >> 4) * 4 + r][j * 16 + (lane & 15)] = acc[i][j][r];
+        if (head_lds) {
+            for (int e = lane; e < 16 * FAR_PTS; e += 64) {
+                const int t = e / FAR_PTS, p = e % FAR_PTS;
+                const int m = m0 + wm + i * 16 + t;
+                Fs[wave][t][p] = m < M ? out.far[((size_t)m * FAR_PTS + p) * 6 + hc] : 0.0f;
+            }
+        }
         __syncthreads();
+        if constexpr (head_lds) {
+            const int n = n0 + wn + lane;
+            const bool train = out.fine != nullptr;
+            const float sc = hc < 3 ? out.scale_e : out.scale_b;
+            const float kf = hc < 3 ? out.ke : out.kmu;
+            const float* stat = hc < 3 ? out.e_stat : out.b_stat;
+            float* dyn = hc < 3 ? out.e_dyn : out.b_dyn;
+            // rows in batches of 8: all loads of a batch (flags, static
+            // field) are issued before its first store
+#pragma unroll
+            for (int tb = 0; tb < 16; tb += 8) {
+                uint64_t cell[8];
+                bool valid[8];
+                uint8_t fl[8];
+                float st[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int m_raw = m0 + wm + i * 16 + tb + u;
+                    valid[u] = m_raw < M && n < N;  // always true for the shapes this path runs
+                    const uint32_t m = valid[u] ? (uint32_t)m_raw : 0u;
+                    const uint32_t X = m % out.nd, Y = (m / out.nd) % out.nd, Z = m / (out.nd * out.nd);
+                    cell[u] = (uint64_t)hcell0 + (uint64_t)(X * out.bx) +
+                              ((uint64_t)(Y * out.by) + (uint64_t)(Z * out.bz) * out.ny) * out.nx;
+                }
+                if (!train) {
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        fl[u] = out.flags[cell[u]];
+                        st[u] = stat[(size_t)(hc % 3) * out.ncell + cell[u]];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int t = tb + u;
+                    // separable tri-quadratic interpolation (x, then y, then z)
+                    float far = 0.0f;
+#pragma unroll
+                    for (int zi = 0; zi < 3; zi++) {
+                        float fy = 0.0f;
+#pragma unroll
+                        for (int yi = 0; yi < 3; yi++) {
+                            const float* f = &Fs[wave][t][zi * 9 + yi * 3];
+                            const float fx = __builtin_fmaf(Lx[2], f[2], __builtin_fmaf(Lx[1], f[1], Lx[0] * f[0]));
+                            fy = __builtin_fmaf(Ly[yi], fx, fy);
+                        }
+                        far = __builtin_fmaf(Lz[zi], fy, far);
+                    }
+                    const float v = out.alpha * Cs[wave][t][lane];
+                    if (!valid[u]) continue;
+                    if (train) out.fine[(size_t)hc * out.ncell + cell[u]] = v + far * (out.gain / in_s);
+                    else if ((fl[u] & 0x1F) != 0x01)
+                        dyn[(size_t)(hc % 3) * out.ncell + cell[u]] = (st[u] + in_s * sc * v) + kf * far;
+                }
+            }
+        } else {
 #pragma unroll 1
-        for (int t = 0; t < 16; t++)
-            epilogue<EPI>(out, M, N, m0 + wm + i * 16 + t, n0 + wn + lane, Cs[wave][t][lane]);
+            for (int t = 0; t < 16; t++)
+                epilogue<EPI>(out, M, N, m0 + wm + i * 16 + t, n0 + wn + lane, Cs[wave][t][lane]);
+        }
     }
 }
 
@@ -534,7 +617,7 @@ static inline dim3 grid_for(size_t n) {
 hipError_t gemm_nt(int epi, const __bf16* A, const __bf16* B, int M, int N, int K, int lda, int ldb, const GemmOut& o,
                    hipStream_t s) {
     if (K % BK != 0 || M <= 0 || N <= 0) return hipErrorInvalidValue;
-    if (M >= BM2 && N >= BN2) {
+    if (M >= BM2 && N >= BN2 && (epi != EPI_HEAD || (o.cpb & 63u) == 0)) {
         const dim3 g((N + BN2 - 1) / BN2, (M + BM2 - 1) / BM2);
         if (epi == EPI_F32) hipLaunchKernelGGL(k_gemm_nt_128<EPI_F32>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
         else if (epi == EPI_BF16) hipLaunchKernelGGL(k_gemm_nt_128<EPI_BF16>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
