@@ -53,7 +53,14 @@ typedef struct ion_units {
 
 /* LbmConfig, src/lbm/mod.rs:45-101. graphics_active stands for
  * graphics_config.graphics_active, the only GraphicsConfig field that reaches
- * the kernels (it switches UPDATE_FIELDS on, src/lbm/domain.rs:856). */
+ * the kernels (it switches UPDATE_FIELDS on, src/lbm/domain.rs:856).
+ * ext_charge_inlet has no reference counterpart (the reference has no charge
+ * injection): with ext_magneto_hydro and ext_equilibrium_boudaries it makes
+ * the TYPE_E cells charge sources as well. There the charge density is taken
+ * from the Q buffer (set by the caller, like rho and u of TYPE_E cells),
+ * the D3Q7 charge DDFs are reset to their equilibrium with that charge every
+ * step (no collision), and Q is not written back. It is not part of the save
+ * format (ion_load returns it as 0). */
 typedef struct ion_config {
     int32_t velocity_set;
     int32_t relaxation_time;
@@ -70,7 +77,7 @@ typedef struct ion_config {
     uint8_t ext_subgrid_ecr;
     uint8_t mhd_lod_depth;
     uint8_t graphics_active;
-    uint8_t reserved0;
+    uint8_t ext_charge_inlet;
     float ecr_freq;
     float ecr_field_strength;
     uint64_t run_steps;

@@ -33,6 +33,7 @@ struct LbmArgs {
                           // load-side ones live in scalar registers across the cell
     uint32_t odd;         // t % 2 (esoteric-pull layout)
     uint32_t eqb, vforce, ffield, upd;  // extension switches
+    uint32_t qin;         // charge inlet: TYPE_E cells hold their charge (build extension)
     uint32_t lod_bx;      // LOD block side along x (cells)
     float fx, fy, fz;     // volume force
     float w;              // DEF_W

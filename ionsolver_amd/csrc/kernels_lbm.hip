@@ -259,13 +259,17 @@ __global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
 #pragma unroll
             for (int i = 0; i < 7; i++) rhon_q += qhn[i];
             rhon_q += R(1);
+            // charge inlet (build extension): a TYPE_E cell keeps the charge the
+            // Q buffer holds and its charge DDFs are reset to that equilibrium
+            const bool inlet = is_e && a.qin;
+            if (inlet) rhon_q = *at(Qp, n);
             // charge advection part 2
-            *at(Qp, n) = rhon_q - rhon_e;
+            else *at(Qp, n) = rhon_q - rhon_e;
             R qeq[7];
             a_eq(rhon_q, uxn, uyn, uzn, qeq);
             const R wq = WQ;
 #pragma unroll
-            for (int i = 0; i < 7; i++) qhn[i] = fmaR(R(1) - wq, qhn[i], wq * qeq[i]);
+            for (int i = 0; i < 7; i++) qhn[i] = inlet ? qeq[i] : fmaR(R(1) - wq, qhn[i], wq * qeq[i]);
             store_ddfs<19, 7, S>((T*)a.fqi, a.N_store, n, b, a.odd, qhn);
             // electron gas part 2: F = -rho_e (E + u_e x B), Guo forcing, SRT
             const R crx = uyn_e * Bz - uzn_e * By;

@@ -206,6 +206,7 @@ ion::LbmArgs lbm_args(ion_handle* h) {
     a.vforce = h->cfg.ext_volume_force ? 1u : 0u;
     a.ffield = h->cfg.ext_force_field ? 1u : 0u;
     a.upd = h->cfg.graphics_active ? 1u : 0u;
+    a.qin = h->cfg.ext_charge_inlet ? 1u : 0u;
     a.lod_bx = h->lod_bx;
     a.fx = h->fx;
     a.fy = h->fy;
@@ -357,6 +358,8 @@ int validate(const ion_config& c) {
                                          "per-direction array with 32-bit byte offsets");
     if (c.ext_subgrid_ecr) return fail(ION_ERR_UNSUPPORTED, "ext_subgrid_ecr is not supported");
     if (c.ext_force_field && !c.ext_volume_force) return fail(ION_ERR_INVALID, "ext_force_field needs ext_volume_force");
+    if (c.ext_charge_inlet && !(c.ext_magneto_hydro && c.ext_equilibrium_boudaries))
+        return fail(ION_ERR_INVALID, "ext_charge_inlet needs ext_magneto_hydro and ext_equilibrium_boudaries");
     if (c.ext_magneto_hydro) {
         if (c.velocity_set == ION_D2Q9) return fail(ION_ERR_UNSUPPORTED, "ext_magneto_hydro needs a 3D velocity set");
         if (!c.ext_volume_force) return fail(ION_ERR_INVALID, "ext_magneto_hydro needs ext_volume_force");

@@ -98,7 +98,7 @@ class _Config(ctypes.Structure):
         ("ext_equilibrium_boudaries", ctypes.c_uint8), ("ext_volume_force", ctypes.c_uint8),
         ("ext_force_field", ctypes.c_uint8), ("ext_magneto_hydro", ctypes.c_uint8),
         ("ext_subgrid_ecr", ctypes.c_uint8), ("mhd_lod_depth", ctypes.c_uint8),
-        ("graphics_active", ctypes.c_uint8), ("reserved0", ctypes.c_uint8),
+        ("graphics_active", ctypes.c_uint8), ("ext_charge_inlet", ctypes.c_uint8),
         ("ecr_freq", ctypes.c_float), ("ecr_field_strength", ctypes.c_float),
         ("run_steps", ctypes.c_uint64),
     ]
@@ -263,6 +263,8 @@ class LbmConfig:
         self.ext_force_field = False
         self.ext_magneto_hydro = False
         self.ext_subgrid_ecr = False
+        # no reference counterpart: TYPE_E cells also hold their charge (Q) fixed
+        self.ext_charge_inlet = False
         self.mhd_lod_depth = 4
         self.ecr_freq = 0.0
         self.ecr_field_strength = 0.0
@@ -286,6 +288,7 @@ class LbmConfig:
         c.ext_force_field = int(bool(self.ext_force_field))
         c.ext_magneto_hydro = int(bool(self.ext_magneto_hydro))
         c.ext_subgrid_ecr = int(bool(self.ext_subgrid_ecr))
+        c.ext_charge_inlet = int(bool(self.ext_charge_inlet))
         c.mhd_lod_depth = self.mhd_lod_depth
         c.graphics_active = int(bool(self.graphics_config.graphics_active))
         c.ecr_freq = self.ecr_freq
@@ -310,6 +313,7 @@ class LbmConfig:
         cfg.ext_force_field = bool(c.ext_force_field)
         cfg.ext_magneto_hydro = bool(c.ext_magneto_hydro)
         cfg.ext_subgrid_ecr = bool(c.ext_subgrid_ecr)
+        cfg.ext_charge_inlet = bool(c.ext_charge_inlet)
         cfg.mhd_lod_depth = c.mhd_lod_depth
         cfg.graphics_config.graphics_active = bool(c.graphics_active)
         cfg.ecr_freq, cfg.ecr_field_strength, cfg.run_steps = c.ecr_freq, c.ecr_field_strength, c.run_steps

@@ -154,15 +154,17 @@ def _loop_field(r, z, a, wire):
 
 
 def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magnet_r=350.0, b_center=0.05,
-             u_inlet=0.02, n_blobs=8, sigma=12.0, amp=0.05, seed=1234, log=None):
+             u_inlet=0.02, q_inlet=0.02, n_blobs=8, sigma=12.0, amp=0.05, seed=1234, log=None):
     """Synthetic plasma-thruster workload: a z-aligned quartz tube (solid
     shell, flags 0x01) of radius tube_r, a ring magnet (current loop of
     radius magnet_r at mid-height, analytic field scaled to b_center at its
     centre) as B_stat, an inlet plane (flags 0x02, equilibrium boundary) at
     z = 0 inside the tube with u = (0, 0, u_inlet), the tube filled with
-    fluid moving at u_inlet and seeded with Gaussian charge blobs. Lengths
-    are given for n = 1024 and scaled with n. Arrays are built one z-layer at
-    a time. Returns (cfg, arrays, meta)."""
+    fluid moving at u_inlet and seeded with Gaussian charge blobs. With
+    q_inlet set (None switches it off) the inlet also injects charge: the
+    charge inlet extension holds the inlet cells at charge density q_inlet.
+    Lengths are given for n = 1024 and scaled with n. Arrays are built one
+    z-layer at a time. Returns (cfg, arrays, meta)."""
     from .lbm import FloatType as FT
     ft = FT.FP16C if float_type is None else float_type
     sc = n / 1024.0
@@ -176,6 +178,7 @@ def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magne
     cfg.ext_volume_force = True
     cfg.ext_magneto_hydro = True
     cfg.ext_equilibrium_boudaries = True
+    cfg.ext_charge_inlet = q_inlet is not None
     cfg.mhd_lod_depth = lod_depth
     N = n ** 3
     c = 0.5 * n
@@ -210,6 +213,8 @@ def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magne
         qz = np.zeros((n, n))
         for g, (bx, by, bz, a) in zip(gxy, blobs):
             qz += g * np.exp(-((z - bz) ** 2) / s2)
+        if z == 0 and q_inlet is not None:
+            qz = np.full_like(qz, q_inlet)
         q[sl] = np.where(inside, qz, 0.0).ravel()
         brl, bzl = _loop_field(r, np.full_like(r, z - c), rm, 2.0)
         with np.errstate(divide="ignore", invalid="ignore"):
@@ -221,5 +226,6 @@ def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magne
         if log and z % 128 == 127:
             log(f"thruster setup: layer {z + 1}/{n}")
     arrays = dict(flags=flags, u=u, q=q, b_stat=b)
-    meta = dict(tube_r=rt, wall=th, magnet_r=rm, b_center=b_center, u_inlet=u_inlet, blobs=len(blobs), seed=seed)
+    meta = dict(tube_r=rt, wall=th, magnet_r=rm, b_center=b_center, u_inlet=u_inlet, q_inlet=q_inlet,
+                blobs=len(blobs), seed=seed)
     return cfg, arrays, meta

@@ -532,11 +532,14 @@ struct Sim : SimBase {
             R rhon_q = R(0);
             for (int i = 0; i < 7; i++) rhon_q += qhn[i];
             rhon_q += R(1);
+            /* charge inlet (ORC_EXT_CHARGE_INLET): the cell's charge is held */
+            const bool inlet = is_e && ext(ORC_EXT_CHARGE_INLET);
+            if (inlet) rhon_q = q[n];
             /* charge advection part 2 (:633-637) */
-            q[n] = rhon_q - rhon_e;
+            else q[n] = rhon_q - rhon_e;
             R qeq[7];
             a_eq<R>(rhon_q, uxn, uyn, uzn, qeq);
-            for (int i = 0; i < 7; i++) qhn[i] = std::fma(R(1) - WQ, qhn[i], WQ * qeq[i]);
+            for (int i = 0; i < 7; i++) qhn[i] = inlet ? qeq[i] : std::fma(R(1) - WQ, qhn[i], WQ * qeq[i]);
             store<7>(n, qhn, fqi.data(), j7, tt);
             /* electron gas part 2 (:641-656): F = -rho_e (E + u_e x B) */
             const R crx = uyn_e * Bz - uzn_e * By;

@@ -43,6 +43,11 @@ enum {
     ORC_EXT_FORCE_FIELD = 1 << 2,
     ORC_EXT_MAGNETO_HYDRO = 1 << 3,
     ORC_EXT_UPDATE_FIELDS = 1 << 4,
+    /* Charge inlet, a build-defined extension with no reference counterpart:
+     * TYPE_E cells (with EQUILIBRIUM_BOUNDARIES and MAGNETO_HYDRO) take the
+     * charge density from q[n], reset their D3Q7 DDFs to a_eq(q[n], u) with
+     * no collision and leave q[n] unwritten. */
+    ORC_EXT_CHARGE_INLET = 1 << 5,
 };
 
 /* Storage formats: discriminants of FloatType (src/lbm/types.rs:75-83), plus

@@ -40,6 +40,8 @@ def oracle_for(cfg, float_type=None):
         ext |= O.EXT_MAGNETO_HYDRO
     if cfg.graphics_config.graphics_active:
         ext |= O.EXT_UPDATE_FIELDS
+    if cfg.ext_charge_inlet:
+        ext |= O.EXT_CHARGE_INLET
     ft = int(cfg.float_type) if float_type is None else float_type
     return O.OracleSim(cfg.n_x, cfg.n_y, cfg.n_z, velocity_set=int(cfg.velocity_set),
                        relaxation_time=int(cfg.relaxation_time), float_type=ft, lod_depth=cfg.mhd_lod_depth,
@@ -80,7 +82,8 @@ def write_both(gpu, orc, arrays):
 
 
 def make_config(nx=16, ny=16, nz=16, velocity_set=2, float_type=2, relaxation_time=0, mhd=False, lod_depth=2,
-                eqb=True, vforce=True, ffield=False, graphics=True, force=(1e-5, -2e-5, 3e-5), nu=0.02, units=None):
+                eqb=True, vforce=True, ffield=False, graphics=True, force=(1e-5, -2e-5, 3e-5), nu=0.02, units=None,
+                charge_inlet=False):
     cfg = I.LbmConfig()
     cfg.velocity_set = I.VelocitySet(velocity_set)
     cfg.float_type = I.FloatType(float_type)
@@ -92,6 +95,7 @@ def make_config(nx=16, ny=16, nz=16, velocity_set=2, float_type=2, relaxation_ti
     cfg.ext_force_field = ffield
     cfg.ext_magneto_hydro = mhd
     cfg.mhd_lod_depth = lod_depth
+    cfg.ext_charge_inlet = charge_inlet
     cfg.graphics_config.graphics_active = graphics
     if vforce or mhd:
         cfg.f_x, cfg.f_y, cfg.f_z = force

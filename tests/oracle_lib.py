@@ -17,6 +17,7 @@ EXT_VOLUME_FORCE = 1 << 1
 EXT_FORCE_FIELD = 1 << 2
 EXT_MAGNETO_HYDRO = 1 << 3
 EXT_UPDATE_FIELDS = 1 << 4
+EXT_CHARGE_INLET = 1 << 5
 
 FP16S, FP16C, FP32, FP64 = 0, 1, 2, 3
 D2Q9, D3Q15, D3Q19, D3Q27 = 0, 1, 2, 3

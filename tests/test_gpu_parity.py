@@ -53,6 +53,17 @@ def test_mhd_lod_depth0_and_d3q15():
     assert compare_all(gpu, orc, info) == []
 
 
+@pytest.mark.parametrize("float_type,trt", [(2, 0), (1, 0), (2, 1)])
+def test_mhd_charge_inlet(float_type, trt):
+    """Charge inlet extension: TYPE_E cells hold their charge and reset
+    their charge DDFs to equilibrium, every buffer exact."""
+    rng = np.random.default_rng(450 + float_type + 5 * trt)
+    gpu, orc, info = make_pair(rng, nx=16, ny=16, nz=16, mhd=True, lod_depth=2, float_type=float_type,
+                               relaxation_time=trt, charge_inlet=True)
+    run_both(gpu, orc, 4)
+    assert compare_all(gpu, orc, info) == []
+
+
 def test_device_codecs_match_oracle():
     import oracle_lib as O
     import ionsolver_amd as I

@@ -241,3 +241,60 @@ def test_electron_gas_starts_at_rounding_residue(ft, residue):
     for i in range(1, 19):
         r = np.float32(r + e[i, 0])
     assert np.float32(r + np.float32(1.0)) == np.float32(residue)
+
+
+def inlet_ring(nx, ft, q_in, u0):
+    """A ring of nx x 4 x 4 cells, periodic, with the plane x = 0 a TYPE_E
+    charge inlet holding q_in; everything else fluid, initially uncharged."""
+    n = (nx, 4, 4)
+    orc = O.OracleSim(*n, lod_depth=2, float_type=ft, w=1.5, ke=0.0, kmu=0.0, kkge=-1.0, wq=0.4,
+                      ext=MHD_EXT | O.EXT_EQUILIBRIUM_BOUNDARIES | O.EXT_CHARGE_INLET)
+    cells = nx * 16
+    x = np.tile(np.arange(nx), 16)
+    inlet = x == 0
+    orc.buf("flags")[:] = np.where(inlet, 0x02, 0x00)
+    q = np.zeros(cells)
+    q[inlet] = q_in
+    orc.buf("q")[:] = q
+    orc.buf("u")[:] = np.repeat(np.asarray(u0, np.float64), cells)
+    orc.initialize()
+    orc.buf("ei")[:] = 0  # electron gas at rest with density 1: Q = rho_q - 1 in fluid cells
+    return orc, inlet
+
+
+@pytest.mark.parametrize("u0", [(0.0, 0.0, 0.0), (0.05, 0.0, 0.0)])
+def test_charge_inlet_fills_the_ring(u0):
+    """With one inlet plane in a periodic ring the only steady state of
+    advection-diffusion is the inlet's charge everywhere: the inlet cells keep
+    q_in, the total charge grows monotonically and converges to q_in per cell
+    (the slowest mode decays by about exp(-0.0066 t) at rest)."""
+    nx, q_in = 24, 0.3
+    orc, inlet = inlet_ring(nx, O.FP64, q_in, u0)
+    tot = []
+    for t in range(2200):
+        orc.do_time_step()
+        if t % 100 == 0:
+            rq = np.where(inlet, orc.buf("q"), orc.buf("q") + 1.0)
+            tot.append(rq.sum())
+    assert np.all(orc.buf("q")[inlet] == q_in)
+    assert np.all(np.diff(tot) > 0)
+    rq = orc.buf("q")[~inlet] + 1.0
+    assert np.max(np.abs(rq - q_in)) < 1e-6
+
+
+def test_charge_inlet_off_conserves_charge():
+    """Planting the same TYPE_E plane without the inlet extension conserves
+    the total charge instead (the TYPE_E charge advects like fluid)."""
+    nx, q_in = 24, 0.3
+    orc = O.OracleSim(nx, 4, 4, lod_depth=2, float_type=O.FP64, w=1.5, ke=0.0, kmu=0.0, kkge=-1.0, wq=0.4,
+                      ext=MHD_EXT | O.EXT_EQUILIBRIUM_BOUNDARIES)
+    x = np.tile(np.arange(nx), 16)
+    orc.buf("flags")[:] = np.where(x == 0, 0x02, 0x00)
+    orc.buf("q")[:] = np.where(x == 0, q_in, 0.0)
+    orc.initialize()
+    orc.buf("ei")[:] = 0
+    orc.do_time_step()
+    t0 = (orc.buf("q") + 1.0).sum()
+    for _ in range(200):
+        orc.do_time_step()
+    assert abs((orc.buf("q") + 1.0).sum() - t0) < 1e-12 * abs(t0)

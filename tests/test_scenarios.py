@@ -43,4 +43,5 @@ def test_thruster_geometry():
     # mirror symmetry of the loop field about its plane: B_z even, B_x odd in z - n/2
     np.testing.assert_allclose(b[2, n // 2 + 5], b[2, n // 2 - 5], rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(b[0, n // 2 + 5], -b[0, n // 2 - 5], rtol=1e-5, atol=1e-9)
-    assert cfg.ext_equilibrium_boudaries and cfg.ext_magneto_hydro
+    assert cfg.ext_equilibrium_boudaries and cfg.ext_magneto_hydro and cfg.ext_charge_inlet
+    assert np.all(q[0][r < meta["tube_r"]] == np.float32(meta["q_inlet"]))
