@@ -262,6 +262,46 @@ int ion_surrogate_write_params(ion_handle* h, const float* src, uint64_t n);
  * ------------------------------------------------------------------------ */
 int ion_hbm_copy_probe(uint64_t bytes, uint32_t reps, double* best_ms, double* gb_per_s);
 
+/* ------------------------------------------------------------------------
+ * Meshes and static fields (src/mesh.rs, src/lbm/mod.rs:284-314).
+ * Meshes belong to the handle; `index` is the position in Lbm::meshes.
+ * ------------------------------------------------------------------------ */
+/* ModelType, src/mesh.rs:9-14; flags 0x01, 0x11, 0x09, 0x05 (mesh.rs:309-314). */
+typedef enum {
+    ION_MODEL_SOLID = 0,
+    ION_MODEL_MAGNET = 1,      /* magnetisation (vx, vy, vz) in A/m */
+    ION_MODEL_CHARGED = 2,     /* charge vx in C per cell */
+    ION_MODEL_CHARGED_ECR = 3  /* charge vx in C per cell */
+} ion_model_type;
+
+/* Lbm::import_mesh, src/mesh.rs:234-240: binary STL read, rotated by
+ * (rx, ry, rz) degrees, scaled by len_si_lu(scale), placed at origin
+ * (ox, oy, oz). ION_ERR_IO if unreadable, ION_ERR_FORMAT if not a binary
+ * STL of 84 + 50 T bytes with T > 0 (the reference panics). */
+int ion_import_mesh(ion_handle* h, const char* path, float scale, float ox, float oy, float oz, float rx, float ry,
+                    float rz, uint32_t* index);
+/* Lbm::import_mesh_reposition, src/mesh.rs:248-254: centred at (cx, cy, cz),
+ * longest side `size` cells (0: fit the lattice box, < 0: scale factor -size). */
+int ion_import_mesh_reposition(ion_handle* h, const char* path, float cx, float cy, float cz, float rx, float ry,
+                               float rz, float size, uint32_t* index);
+/* The imported triangles (vertices[9 T]: p0, p1, p2) and bounds
+ * (p_min, p_max, center: 9 floats); either pointer may be null. */
+int ion_mesh_get(ion_handle* h, uint32_t index, uint32_t* triangles, float* vertices, float* bounds);
+/* Lbm::voxelise_mesh, src/mesh.rs:256-343 and voxelize_mesh,
+ * src/kernels/sim_kernels.cl:1150-1231: cells inside the mesh get
+ * flags = (flags & 0xE0) | flag of the model type; with ext_magneto_hydro a
+ * magnet writes its magnetisation (lattice units) into B_dyn and a charged
+ * model its charge into B_dyn.x, where precompute reads them. Run before
+ * ion_initialize (which sets B_dyn = B_stat). */
+int ion_voxelise_mesh(ion_handle* h, uint32_t index, int model_type, float vx, float vy, float vz);
+/* Lbm::precompute_B, src/lbm/mod.rs:284-297 (psi_from_mesh +
+ * static_b_from_mesh): B_stat += -mu0 grad psi of the magnetised cells.
+ * psi is left in E_dyn (padded lattice) as in the reference. Synchronous. */
+int ion_precompute_b(ion_handle* h);
+/* Lbm::precompute_E, src/lbm/mod.rs:302-314 (static_e_from_mesh): E_stat +=
+ * Coulomb field of the charged cells. Synchronous. */
+int ion_precompute_e(ion_handle* h);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* ion_last_error(void);
 

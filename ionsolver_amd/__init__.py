@@ -4,8 +4,10 @@ The compute path is libionsolver_hip.so (hand-written HIP kernels behind the C
 ABI of include/ionsolver.h); this package is the host-side mirror of the
 reference's Lbm/LbmConfig API over that ABI, plus scenario setups.
 """
-from .lbm import (Field, FieldMode, FieldSolver, FloatType, GraphicsConfig, IonError, Lbm, LbmConfig, Propellant,
-                  RelaxationTime, Units, VelocitySet, device_codec, hbm_copy_probe, load_library, LIB_PATH, ABI)
+from .lbm import (Field, FieldMode, FieldSolver, FloatType, GraphicsConfig, IonError, Lbm, LbmConfig, ModelType,
+                  Propellant, RelaxationTime, Units, VelocitySet, device_codec, hbm_copy_probe, load_library, LIB_PATH,
+                  ABI)
 
-__all__ = ["Field", "FieldMode", "FieldSolver", "FloatType", "GraphicsConfig", "IonError", "Lbm", "LbmConfig", "Propellant",
+__all__ = ["Field", "FieldMode", "FieldSolver", "FloatType", "GraphicsConfig", "IonError", "Lbm", "LbmConfig", "ModelType",
+           "Propellant",
            "RelaxationTime", "Units", "VelocitySet", "device_codec", "hbm_copy_probe", "load_library", "LIB_PATH", "ABI"]

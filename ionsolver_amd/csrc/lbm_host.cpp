@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "internal.hpp"
 #include "ionsolver.h"
 #include "kernels.hpp"
 #include "surrogate_host.hpp"
@@ -103,6 +104,8 @@ struct ion_handle {
     uint64_t field_steps = 0;
     uint32_t fallback_left = 0;
     ion_surrogate_stats sstats{};
+    // imported meshes (mesh_host.cpp)
+    ion::MeshSet* meshes = nullptr;
 
     size_t ddf_elem() const { return store == ion::STORE_FP64 ? 8 : (store == ion::STORE_FP32 ? 4 : 2); }
     bool f64() const { return store == ion::STORE_FP64; }
@@ -125,6 +128,8 @@ void free_all(ion_handle* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     delete h->net;
     h->net = nullptr;
+    ion::mesh_set_destroy(h->meshes);
+    h->meshes = nullptr;
     for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
@@ -398,6 +403,26 @@ extern "C" {
 const char* ion_last_error(void) { return g_err.c_str(); }
 
 __attribute__((visibility("hidden"))) int ion_internal_set_error(int code, const char* msg) { return fail(code, msg); }
+
+__attribute__((visibility("hidden"))) int ion_internal_view(ion_handle* h, ion::HandleView* v) {
+    if (!h) return fail(ION_ERR_INVALID, "null handle");
+    v->stream = h->stream;
+    v->nx = h->cfg.n_x;
+    v->ny = h->cfg.n_y;
+    v->nz = h->cfg.n_z;
+    v->N = h->N;
+    v->mhd = h->mhd;
+    v->f64 = h->f64();
+    v->cfg = &h->cfg;
+    v->kc = &h->kc;
+    v->flags = h->flags;
+    v->e_stat = h->e_stat;
+    v->b_stat = h->b_stat;
+    v->e_dyn = h->e_dyn;
+    v->b_dyn = h->b_dyn;
+    v->meshes = &h->meshes;
+    return ION_OK;
+}
 
 void ion_units_default(ion_units* u) {
     u->m = 1.0f; u->kg = 1.0f; u->s = 1.0f; u->a = 1.0f; u->k = 1.0f;

@@ -132,6 +132,14 @@ class FieldSolver(enum.IntEnum):
     SURROGATE = 1
 
 
+class ModelType(enum.IntEnum):
+    """ModelType, src/mesh.rs:9-14."""
+    Solid = 0
+    Magnet = 1      # magnetisation (x, y, z) in A/m
+    Charged = 2     # charge per cell in C
+    ChargedECR = 3  # charge per cell in C
+
+
 # Every entry point of include/ionsolver.h: name -> (restype, argtypes)
 _VP, _I, _U64, _F, _D, _SZ = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_float, ctypes.c_double, ctypes.c_size_t
 ABI = {
@@ -169,6 +177,12 @@ ABI = {
     "ion_set_field_solver": (_I, [_VP, _I, ctypes.c_uint32, _D]),
     "ion_surrogate_set_learning_rate": (_I, [_VP, ctypes.c_float, ctypes.c_float]),
     "ion_hbm_copy_probe": (_I, [_U64, ctypes.c_uint32, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "ion_import_mesh": (_I, [_VP, ctypes.c_char_p] + [_F] * 7 + [ctypes.POINTER(ctypes.c_uint32)]),
+    "ion_import_mesh_reposition": (_I, [_VP, ctypes.c_char_p] + [_F] * 7 + [ctypes.POINTER(ctypes.c_uint32)]),
+    "ion_mesh_get": (_I, [_VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), _VP, _VP]),
+    "ion_voxelise_mesh": (_I, [_VP, ctypes.c_uint32, _I, _F, _F, _F]),
+    "ion_precompute_b": (_I, [_VP]),
+    "ion_precompute_e": (_I, [_VP]),
     "ion_surrogate_get_stats": (_I, [_VP, ctypes.POINTER(_SurStats)]),
     "ion_surrogate_param_count": (_I, [_VP, ctypes.POINTER(_U64)]),
     "ion_surrogate_read_params": (_I, [_VP, _I, _VP, _U64]),
@@ -479,6 +493,43 @@ class Lbm:
     def set_surrogate_params(self, params):
         arr = np.ascontiguousarray(params, np.float32)
         _check(self._lib.ion_surrogate_write_params(self._h, arr.ctypes.data_as(ctypes.c_void_p), arr.size))
+
+    # meshes and static fields (src/mesh.rs, src/lbm/mod.rs:284-314)
+    def import_mesh(self, path, scale, ox, oy, oz, rx=0.0, ry=0.0, rz=0.0):
+        """Lbm::import_mesh (src/mesh.rs:234); returns the mesh index."""
+        i = ctypes.c_uint32()
+        _check(self._lib.ion_import_mesh(self._h, os.fsencode(path), scale, ox, oy, oz, rx, ry, rz, ctypes.byref(i)))
+        return i.value
+
+    def import_mesh_reposition(self, path, cx, cy, cz, rx, ry, rz, size):
+        """Lbm::import_mesh_reposition (src/mesh.rs:248); returns the mesh index."""
+        i = ctypes.c_uint32()
+        _check(self._lib.ion_import_mesh_reposition(self._h, os.fsencode(path), cx, cy, cz, rx, ry, rz, size,
+                                                    ctypes.byref(i)))
+        return i.value
+
+    def mesh(self, index):
+        """(vertices [T, 3, 3] f32, bounds dict) of an imported mesh."""
+        t = ctypes.c_uint32()
+        _check(self._lib.ion_mesh_get(self._h, index, ctypes.byref(t), None, None))
+        v = np.empty((t.value, 3, 3), np.float32)
+        b = np.empty(9, np.float32)
+        _check(self._lib.ion_mesh_get(self._h, index, ctypes.byref(t), v.ctypes.data_as(ctypes.c_void_p),
+                                      b.ctypes.data_as(ctypes.c_void_p)))
+        return v, dict(p_min=b[0:3], p_max=b[3:6], center=b[6:9])
+
+    def voxelise_mesh(self, index, model_type, value=(0.0, 0.0, 0.0)):
+        """Lbm::voxelise_mesh (src/mesh.rs:256); value: magnetisation (A/m) or (charge, 0, 0) in C."""
+        v = tuple(value) if np.ndim(value) else (value, 0.0, 0.0)
+        _check(self._lib.ion_voxelise_mesh(self._h, index, int(model_type), *[float(x) for x in v]))
+
+    def precompute_b(self):
+        """Lbm::precompute_B (src/lbm/mod.rs:284)."""
+        _check(self._lib.ion_precompute_b(self._h))
+
+    def precompute_e(self):
+        """Lbm::precompute_E (src/lbm/mod.rs:302)."""
+        _check(self._lib.ion_precompute_e(self._h))
 
 
 def hbm_copy_probe(nbytes=1 << 30, reps=10):

@@ -128,6 +128,19 @@ void orc_rho_u_f64(int vs, const double* f, double* rho_u4);
 int orc_num_threads(void);
 void orc_set_num_threads(int n);
 
+/* Mesh voxelisation and static fields (mesh_oracle.cpp), on raw buffers:
+ * voxelize_mesh (sim_kernels.cl:1150-1231) for the triangles tri[9 T] with
+ * mesh bounds pmin/pmax (widened by 2 inside, src/mesh.rs:283-302); b_dyn
+ * may be NULL (no MAGNETO_HYDRO). */
+void orc_voxelize(uint32_t nx, uint32_t ny, uint32_t nz, uint8_t* flags, float* b_dyn, const float* tri, uint32_t T,
+                  const float* pmin, const float* pmax, uint8_t flag, float mx, float my, float mz);
+/* psi_from_mesh + static_b_from_mesh (:1234-1276): psi[(nx+2)(ny+2)(nz+2)],
+ * B = B_stat, M = B_dyn. */
+void orc_precompute_b(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t* flags, const float* M, float* psi,
+                      float* B, float kmu0);
+/* static_e_from_mesh (:1278-1299): E = E_stat, C = B_dyn. */
+void orc_precompute_e(uint32_t nx, uint32_t ny, uint32_t nz, const uint8_t* flags, const float* C, float* E, float ke);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,10 @@ def lib():
         L.orc_a_eq.argtypes = [f32, f32, f32, f32, vp]
         L.orc_num_threads.restype = i32
         L.orc_set_num_threads.argtypes = [i32]
+        u32, u8 = ctypes.c_uint32, ctypes.c_uint8
+        L.orc_voxelize.argtypes = [u32, u32, u32, vp, vp, vp, u32, vp, vp, u8, f32, f32, f32]
+        L.orc_precompute_b.argtypes = [u32, u32, u32, vp, vp, vp, vp, f32]
+        L.orc_precompute_e.argtypes = [u32, u32, u32, vp, vp, vp, f32]
         _lib = L
     return _lib
 
@@ -246,3 +250,40 @@ def set_num_threads(n):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+# mesh voxelisation and static fields (oracle/mesh_oracle.cpp), in place on numpy buffers
+MODEL_FLAG = {0: 0x01, 1: 0x11, 2: 0x09, 3: 0x05}  # src/mesh.rs:309-314
+
+
+def voxelize(shape, flags, b_dyn, tri, p_min, p_max, flag, m=(0.0, 0.0, 0.0)):
+    """voxelize_mesh on flags (uint8 [N]) and b_dyn (float32 [3N] or None)."""
+    nx, ny, nz = shape
+    tri = np.ascontiguousarray(tri, np.float32).reshape(-1)
+    lo = np.ascontiguousarray(p_min, np.float32)
+    hi = np.ascontiguousarray(p_max, np.float32)
+    assert flags.dtype == np.uint8 and flags.flags.c_contiguous and flags.size == nx * ny * nz
+    bd = None
+    if b_dyn is not None:
+        assert b_dyn.dtype == np.float32 and b_dyn.flags.c_contiguous and b_dyn.size == 3 * flags.size
+        bd = _ptr(b_dyn)
+    lib().orc_voxelize(nx, ny, nz, _ptr(flags), bd, _ptr(tri), tri.size // 9, _ptr(lo), _ptr(hi), flag,
+                       float(m[0]), float(m[1]), float(m[2]))
+
+
+def precompute_b(shape, flags, b_dyn, b_stat, kmu0):
+    """psi_from_mesh + static_b_from_mesh; returns psi on the padded lattice."""
+    nx, ny, nz = shape
+    psi = np.zeros((nx + 2) * (ny + 2) * (nz + 2), np.float32)
+    for a in (b_dyn, b_stat):
+        assert a.dtype == np.float32 and a.flags.c_contiguous and a.size == 3 * flags.size
+    lib().orc_precompute_b(nx, ny, nz, _ptr(flags), _ptr(b_dyn), _ptr(psi), _ptr(b_stat), float(kmu0))
+    return psi
+
+
+def precompute_e(shape, flags, b_dyn, e_stat, ke):
+    """static_e_from_mesh (charges read from the x block of b_dyn)."""
+    nx, ny, nz = shape
+    for a in (b_dyn, e_stat):
+        assert a.dtype == np.float32 and a.flags.c_contiguous and a.size == 3 * flags.size
+    lib().orc_precompute_e(nx, ny, nz, _ptr(flags), _ptr(b_dyn), _ptr(e_stat), float(ke))
