@@ -5,9 +5,11 @@ the domain buffers before ``initialize()``, the way the reference's setups do
 with ``bwrite!`` (src/setup.rs). Arrays are SoA in the reference layout
 (n = x + (y + z*Ny)*Nx, vector fields as [x-block, y-block, z-block]).
 """
+import os
+
 import numpy as np
 
-from .lbm import FloatType, LbmConfig, VelocitySet
+from .lbm import FloatType, LbmConfig, ModelType, VelocitySet
 
 # Units of the MHD workloads: the reference's setup_field_vis scaling
 # (src/setup.rs:244-246: 128 cells = 0.1 m, 1 m/s, air density) with the
@@ -229,3 +231,40 @@ def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magne
     meta = dict(tube_r=rt, wall=th, magnet_r=rm, b_center=b_center, u_inlet=u_inlet, q_inlet=q_inlet,
                 blobs=len(blobs), seed=seed)
     return cfg, arrays, meta
+
+
+# setup_deeva_test, src/setup.rs:395-453: the thruster meshes in the order the
+# reference imports and voxelises them (file, origin, model, magnetisation
+# in A/m or charge per cell in C)
+DEEVA_MESHES = [
+    ("deeva_disk_magnet.stl", (64.001, 0.0, 64.0), ModelType.Magnet, (0.0, 1000000.0, 0.0)),
+    ("deeva_inlet.stl", (64.0, 0.0, 64.0), ModelType.Solid, (0.0, 0.0, 0.0)),
+    ("deeva_quartz_tube.stl", (64.001, 0.0, 64.0), ModelType.Solid, (0.0, 0.0, 0.0)),
+    ("deeva_ring_magnet.stl", (64.001, -0.5, 64.0), ModelType.Magnet, (0.0, 500000.0, 0.0)),
+    ("deeva_e_plate1.stl", (64.0, 0.0, 64.0), ModelType.ChargedECR, (0.00000000000021844213 / 2.0, 0.0, 0.0)),
+    ("deeva_e_plate2.stl", (64.0, 0.0, 64.0), ModelType.ChargedECR, (-0.00000000000021844213 / 2.0, 0.0, 0.0)),
+]
+
+
+def deeva(scale=1.0, lod_depth=4):
+    """LbmConfig of setup_deeva_test (src/setup.rs:395-425) with every length
+    scaled by `scale` (1: 128 x 256 x 128). SUBGRID_ECR, which the reference
+    switches on there, is left off: it does not reach the static-field
+    precompute. Returns (cfg, meshes) with the mesh origins scaled alike."""
+    cfg = LbmConfig()
+    cfg.units.set(128.0 * scale, 1.0, 1.0, 1.0, 1.0, 0.1, 1.0, 10e-8, 1.0, 50000.0)
+    cfg.n_x, cfg.n_y, cfg.n_z = int(128 * scale), int(256 * scale), int(128 * scale)
+    cfg.nu = cfg.units.nu_si_lu(0.05)
+    cfg.velocity_set = VelocitySet.D3Q19
+    cfg.ext_volume_force = True
+    cfg.ext_magneto_hydro = True
+    cfg.mhd_lod_depth = lod_depth
+    meshes = [(f, tuple(scale * c for c in o), m, v) for f, o, m, v in DEEVA_MESHES]
+    return cfg, meshes
+
+
+def voxelise_meshes(lbm, meshes, stl_dir):
+    """import_mesh (scale 1) + voxelise_mesh for each (file, origin, model, value)."""
+    for f, origin, model, value in meshes:
+        i = lbm.import_mesh(os.path.join(stl_dir, f), 1.0, *origin)
+        lbm.voxelise_mesh(i, model, value)

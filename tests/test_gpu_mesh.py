@@ -74,8 +74,8 @@ def test_import_errors(tmp_path):
 
 
 def run_both(shape, meshes, lbm_length=None, seed=0, precompute=True):
-    """meshes: list of (kind, args, model_type, value); kind 'file' imports an
-    STL with import_mesh, 'tri' writes the triangles to a file first."""
+    """meshes: list of (STL path, origin, model type, value), imported with
+    import_mesh at scale 1 and voxelised by the library and by the oracle."""
     lbm = make_lbm(shape, lbm_length=lbm_length)
     n = shape[0] * shape[1] * shape[2]
     rng = np.random.default_rng(seed)
@@ -134,13 +134,9 @@ def test_deeva_thruster_at_reduced_resolution():
     """setup_deeva_test (src/setup.rs:395-453) at a quarter of the lattice
     length per axis: 32 x 64 x 32 cells, the meshes imported at the
     reference's placement scaled by 1/4."""
+    from ionsolver_amd.scenarios import deeva
     s = 0.25
-    meshes = [("deeva_disk_magnet", (64.001, 0.0, 64.0), ModelType.Magnet, (0.0, 1000000.0, 0.0)),
-              ("deeva_inlet", (64.0, 0.0, 64.0), ModelType.Solid, (0.0, 0.0, 0.0)),
-              ("deeva_quartz_tube", (64.001, 0.0, 64.0), ModelType.Solid, (0.0, 0.0, 0.0)),
-              ("deeva_ring_magnet", (64.001, -0.5, 64.0), ModelType.Magnet, (0.0, 500000.0, 0.0)),
-              ("deeva_e_plate1", (64.0, 0.0, 64.0), ModelType.ChargedECR, (0.00000000000021844213 / 2.0, 0.0, 0.0)),
-              ("deeva_e_plate2", (64.0, 0.0, 64.0), ModelType.ChargedECR, (-0.00000000000021844213 / 2.0, 0.0, 0.0))]
-    meshes = [(os.path.join(STL, f + ".stl"), tuple(s * c for c in o), m, v) for f, o, m, v in meshes]
+    _, meshes = deeva(s)
+    meshes = [(os.path.join(STL, f), o, m, v) for f, o, m, v in meshes]
     lbm, flags = run_both((32, 64, 32), meshes, lbm_length=128.0 * s)
     assert np.count_nonzero(flags & 0x10) > 1000
