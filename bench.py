@@ -153,7 +153,7 @@ def static_field_leg(scale=1.0):
             "arithmetic": "the reference's: unfused float, correctly rounded sqrt and division, same summation order"}
 
 
-def surrogate_leg(lbm, cfg, samples=4, reps=10, channels=16, train_steps=600, lr=1e-4, lr_head=1e-5):
+def surrogate_leg(lbm, cfg, samples=4, reps=10, channels=16, train_steps=2000, lr=1e-4, lr_head=1e-5):
     """Learned field surrogate against the direct field step on the same
     lattice. Accuracy: the surrogate's relative L2 residual of the dynamic E
     and B against the direct kernel's raw sums, on `samples` random charge
@@ -274,7 +274,8 @@ def main():
     ap.add_argument("--field-mode", default="fast", choices=["exact", "fast"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned field surrogate leg")
-    ap.add_argument("--surrogate-train", type=int, default=600, help="GPU training steps of the surrogate leg")
+    ap.add_argument("--surrogate-train", type=int, default=2000,
+                    help="GPU training steps (fresh snapshots) of the surrogate leg")
     ap.add_argument("--format-errors", action="store_true",
                     help="print the storage-format error table against the FP64 instantiation instead")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
