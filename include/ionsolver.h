@@ -301,6 +301,11 @@ int ion_precompute_b(ion_handle* h);
 /* Lbm::precompute_E, src/lbm/mod.rs:302-314 (static_e_from_mesh): E_stat +=
  * Coulomb field of the charged cells. Synchronous. */
 int ion_precompute_e(ion_handle* h);
+/* Diagnostics (no reference counterpart): the specialised correctly rounded
+ * square root and division of the precompute_B kernel against the general
+ * ones on the current device: every integer in [1, 2^24) for the square root,
+ * div_samples pseudo-random operand pairs of its range for the division. */
+int ion_psi_math_check(uint64_t* sqrt_mismatch, uint64_t* div_mismatch, uint64_t* div_samples);
 
 /* Message of the last failure on this thread ("" if none). */
 const char* ion_last_error(void);

@@ -134,6 +134,13 @@ constexpr uint32_t SEL_ROUNDS = 16;
 constexpr uint32_t SEL_CELLS = SEL_BLOCK * SEL_ROUNDS;  // cells per block
 constexpr uint32_t SCAN_THREADS = 1024;
 
+// Magnetisation components for which k_psi may use the specialised division:
+// 0 or 2^-40 <= |v| <= 2^40 (finite).
+__device__ __forceinline__ bool in_div_range(float v) {
+    const float a = fabsf(v);
+    return a == 0.0f || (a >= 0x1p-40f && a <= 0x1p40f);
+}
+
 __device__ __forceinline__ bool selected(int mode, uint8_t f) {
     return mode == 0 ? (f & TYPE_M) != 0 : ((f & TYPE_F) != 0 || (f & TYPE_C) != 0);
 }
@@ -212,7 +219,13 @@ __global__ __launch_bounds__(SEL_BLOCK) void k_select_scatter(int mode, const ui
             const uint32_t cx = t % nx, cy = t / nx, cz = nn / NXY;
             const uint32_t off = mode == 0 ? 1u : 0u;
             out.pos[k] = make_float4((float)(cx + off), (float)(cy + off), (float)(cz + off), 0.0f);
-            out.val[k] = mode == 0 ? make_float4(m[n], m[N + n], m[2 * N + n], 0.0f) : make_float4(m[n], 0.0f, 0.0f, 0.0f);
+            if (mode == 0) {
+                const float mx = m[n], my = m[N + n], mz = m[2 * N + n];
+                out.val[k] = make_float4(mx, my, mz, 0.0f);
+                if (!(in_div_range(mx) && in_div_range(my) && in_div_range(mz))) atomicOr(out.count + 1, 1u);
+            } else {
+                out.val[k] = make_float4(m[n], 0.0f, 0.0f, 0.0f);
+            }
         }
         for (uint32_t w = 0; w < SEL_BLOCK / 64; w++) run += wsum[w];
     }
@@ -220,9 +233,70 @@ __global__ __launch_bounds__(SEL_BLOCK) void k_select_scatter(int mode, const ui
 
 // ---------------------------------------------------------------------------
 // psi_from_mesh (sim_kernels.cl:1234-1251) over compacted magnet sources.
+//
+// The correctly rounded square root and division the compiler emits
+// (-fhip-fp32-correctly-rounded-divide-sqrt) are, for operands in normal
+// range, v_sqrt_f32 followed by a one-ulp correction either way, and
+// v_rcp_f32 followed by Newton and two residual corrections, wrapped in
+// range scaling (v_div_scale / v_div_fmas) and special-value handling
+// (v_div_fixup, class tests). k_psi's operands stay where the wrapping is the
+// identity: |r|^2 is an exact integer in [1, 2^24) (cell offsets), |r|^3 lies
+// in [1, 2^36], and with every magnetisation component 0 or within
+// [2^-40, 2^40] the dividend r.M is 0 or within [2^-63, 2^54], the quotient
+// normal. So the core sequences alone give the same bits; the scatter pass
+// checks the magnetisation range and k_psi falls back to the general
+// operators otherwise. run_psi_math_check compares the square root over every
+// integer in [1, 2^24) and the division over a large sample.
 // ---------------------------------------------------------------------------
 constexpr uint32_t FLD_BLOCK = 256;
 constexpr uint32_t FLD_TILE = 256;
+
+// correctly rounded sqrt of x in [1, 2^24)
+__device__ __forceinline__ float sqrt_cr(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float s_dn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float s_up = __uint_as_float(__float_as_uint(s) + 1u);
+    const float r_dn = __builtin_fmaf(-s_dn, s, x);
+    const float r_up = __builtin_fmaf(-s_up, s, x);
+    const float t = r_dn <= 0.0f ? s_dn : s;
+    return r_up > 0.0f ? s_up : t;
+}
+
+// correctly rounded a / b for b in [1, 2^36] and a == 0 or |a| in [2^-63, 2^54]
+// (a == 0 may give a zero of the other sign)
+__device__ __forceinline__ float div_cr(float a, float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y0, 1.0f);
+    const float y1 = __builtin_fmaf(e, y0, y0);
+    const float q0 = a * y1;
+    const float r0 = __builtin_fmaf(-b, q0, a);
+    const float q1 = __builtin_fmaf(r0, y1, q0);
+    const float r1 = __builtin_fmaf(-b, q1, a);
+    return __builtin_fmaf(r1, y1, q1);
+}
+
+template <bool SPECIAL>
+__device__ __forceinline__ float psi_tile(const float4* sp, const float4* sv, uint32_t m, float cx, float cy, float cz,
+                                          float psic) {
+    for (uint32_t j = 0; j < m; j++) {
+        const float4 p = sp[j], v = sv[j];
+        const float dx = cx - p.x, dy = cy - p.y, dz = cz - p.z;
+        const float l2 = dx * dx + dy * dy + dz * dz;
+        const float d = dx * v.x + dy * v.y + dz * v.z;
+        if (SPECIAL) {
+            // the own cell (l = 0) adds +0, which leaves psic unchanged: psic
+            // starts at +0 and a round-to-nearest sum is never -0 unless both
+            // terms are
+            const float l = sqrt_cr(l2);
+            const float q = div_cr(d, l * l * l);
+            psic += l2 != 0.0f ? q : 0.0f;
+        } else {
+            const float l = sqrtf(l2);
+            if (!(l == 0.0f)) psic += d / (l * l * l);
+        }
+    }
+    return psic;
+}
 
 __global__ __launch_bounds__(FLD_BLOCK) void k_psi(Sources src, float* psi, uint32_t nx, uint32_t ny, uint32_t nz) {
     __shared__ float4 sp[FLD_TILE];
@@ -232,7 +306,9 @@ __global__ __launch_bounds__(FLD_BLOCK) void k_psi(Sources src, float* psi, uint
     const uint32_t n = blockIdx.x * FLD_BLOCK + threadIdx.x;
     const uint32_t t = n % (NXp * NYp);  // coordinates_sl(n, nx + 2, ny + 2)
     const float cx = (float)(t % NXp), cy = (float)(t / NXp), cz = (float)(n / (NXp * NYp));
-    const uint32_t cnt = *src.count;
+    const uint32_t cnt = src.count[0];
+    // specialised operators need |r|^2 < 2^24: padded extents below 2^11 per axis
+    const bool special = src.count[1] == 0u && NXp < 2048u && NYp < 2048u && nz + 2u < 2048u;
     float psic = 0.0f;
     for (uint32_t b0 = 0; b0 < cnt; b0 += FLD_TILE) {
         const uint32_t m = min(FLD_TILE, cnt - b0);
@@ -242,12 +318,7 @@ __global__ __launch_bounds__(FLD_BLOCK) void k_psi(Sources src, float* psi, uint
             sv[threadIdx.x] = src.val[b0 + threadIdx.x];
         }
         __syncthreads();
-        for (uint32_t j = 0; j < m; j++) {
-            const float4 p = sp[j], v = sv[j];
-            const float dx = cx - p.x, dy = cy - p.y, dz = cz - p.z;
-            const float l = sqrtf(dx * dx + dy * dy + dz * dz);
-            if (!(l == 0.0f)) psic += (dx * v.x + dy * v.y + dz * v.z) / (l * l * l);
-        }
+        psic = special ? psi_tile<true>(sp, sv, m, cx, cy, cz, psic) : psi_tile<false>(sp, sv, m, cx, cy, cz, psic);
     }
     // psic / (4.0f * M_PI): M_PI is a double constant, so the quotient is taken in double
     if (n < Np) psi[n] = (float)((double)psic / (4.0 * 3.14159265358979323846));
@@ -338,6 +409,50 @@ hipError_t launch_select_scatter(int mode, const uint8_t* flags, const float* m,
     hipLaunchKernelGGL(k_select_scatter, dim3(nb), dim3(SEL_BLOCK), 0, s, mode, flags, m, N, nx, ny, block_counts,
                        out);
     return hipGetLastError();
+}
+
+__global__ void k_check_sqrt(unsigned long long* mismatch) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x + 1u;  // 1 .. 2^24 - 1
+    if (i >= (1u << 24)) return;
+    const float x = (float)i;
+    if (__float_as_uint(sqrt_cr(x)) != __float_as_uint(sqrtf(x))) atomicAdd(mismatch, 1ull);
+}
+
+// pseudo-random (a, b) over the specialised ranges, with b = l^3 of an integer l^2
+__global__ void k_check_div(unsigned long long* mismatch, uint32_t seed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t h = i * 0x9E3779B9u ^ seed;
+    h ^= h >> 16; h *= 0x7FEB352Du; h ^= h >> 15; h *= 0x846CA68Bu; h ^= h >> 16;
+    uint32_t g = h * 0x2C1B3C6Du + 0x297A2D39u;
+    g ^= g >> 15; g *= 0x27D4EB2Du; g ^= g >> 13;
+    const float l2 = (float)(1u + (g % ((1u << 24) - 1u)));
+    const float l = sqrtf(l2);
+    const float b = l * l * l;
+    // a: random sign, exponent in [-63, 54], random mantissa
+    const uint32_t e = 127u - 63u + (h % 118u);
+    const float a = __uint_as_float(((h << 8) & 0x80000000u) | (e << 23) | (g & 0x7FFFFFu));
+    if (__float_as_uint(div_cr(a, b)) != __float_as_uint(a / b)) atomicAdd(mismatch, 1ull);
+}
+
+hipError_t run_psi_math_check(uint64_t* sqrt_mismatch, uint64_t* div_mismatch, uint64_t* div_samples) {
+    unsigned long long* d = nullptr;
+    hipError_t e = hipMalloc((void**)&d, 2 * sizeof(unsigned long long));
+    if (e != hipSuccess) return e;
+    e = hipMemset(d, 0, 2 * sizeof(unsigned long long));
+    const uint32_t B = 256;
+    const uint32_t rounds = 16, per = 1u << 24;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_check_sqrt, dim3(((1u << 24) + B - 1) / B), dim3(B), 0, 0, d);
+        for (uint32_t r = 0; r < rounds; r++) hipLaunchKernelGGL(k_check_div, dim3(per / B), dim3(B), 0, 0, d + 1, r * 7919u + 1u);
+        e = hipGetLastError();
+    }
+    unsigned long long h[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    *sqrt_mismatch = h[0];
+    *div_mismatch = h[1];
+    *div_samples = (uint64_t)rounds * per;
+    return e;
 }
 
 hipError_t launch_psi(const Sources& src, float* psi, uint32_t nx, uint32_t ny, uint32_t nz, hipStream_t s) {

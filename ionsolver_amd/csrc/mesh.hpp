@@ -40,7 +40,10 @@ struct VoxArgs {
 struct Sources {
     float4* pos;
     float4* val;
-    const uint32_t* count;  // [1] number of sources (device)
+    // count[0]: number of sources; count[1]: nonzero if a magnet's
+    // magnetisation lies outside the range of the specialised division
+    // (the scatter pass sets it; cleared by the caller before)
+    uint32_t* count;
 };
 
 hipError_t launch_voxelize(const VoxArgs& a, hipStream_t s);
@@ -55,6 +58,10 @@ hipError_t launch_select_count(int mode, const uint8_t* flags, uint64_t N, uint3
                                hipStream_t s);
 hipError_t launch_select_scatter(int mode, const uint8_t* flags, const float* m, uint64_t N, uint32_t nx, uint32_t ny,
                                  const uint32_t* block_counts, Sources out, hipStream_t s);
+
+// Exhaustive / sampled device checks of the specialised correctly rounded
+// square root and division of k_psi against the compiler's general ones.
+hipError_t run_psi_math_check(uint64_t* sqrt_mismatch, uint64_t* div_mismatch, uint64_t* div_samples);
 
 // psi over the lattice padded by one cell on every side
 // ((nx+2)(ny+2)(nz+2) values), from the magnet sources.

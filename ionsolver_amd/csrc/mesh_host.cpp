@@ -203,7 +203,8 @@ struct SourceList {
 int select_sources(const HandleView& v, int mode, SourceList* L) {
     const uint32_t nb = mesh::select_blocks(v.N);
     HIPCHK(hipMalloc((void**)&L->counts, (size_t)(nb + 1) * 4));
-    HIPCHK(hipMalloc((void**)&L->total, 4));
+    HIPCHK(hipMalloc((void**)&L->total, 8));  // count, magnetisation-range flag
+    HIPCHK(hipMemsetAsync(L->total, 0, 8, v.stream));
     HIPCHK(mesh::launch_select_count(mode, v.flags, v.N, L->counts, L->total, v.stream));
     HIPCHK(hipMemcpyAsync(&L->n, L->total, 4, hipMemcpyDeviceToHost, v.stream));
     HIPCHK(hipStreamSynchronize(v.stream));
@@ -357,6 +358,12 @@ int ion_precompute_b(ion_handle* h) {
     if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(ION_ERR_DEVICE, std::string("precompute_B: ") + hipGetErrorString(e));
+    return ION_OK;
+}
+
+int ion_psi_math_check(uint64_t* sqrt_mismatch, uint64_t* div_mismatch, uint64_t* div_samples) {
+    if (!sqrt_mismatch || !div_mismatch || !div_samples) return fail(ION_ERR_INVALID, "null argument");
+    HIPCHK(mesh::run_psi_math_check(sqrt_mismatch, div_mismatch, div_samples));
     return ION_OK;
 }
 

@@ -183,6 +183,7 @@ ABI = {
     "ion_voxelise_mesh": (_I, [_VP, ctypes.c_uint32, _I, _F, _F, _F]),
     "ion_precompute_b": (_I, [_VP]),
     "ion_precompute_e": (_I, [_VP]),
+    "ion_psi_math_check": (_I, [ctypes.POINTER(_U64)] * 3),
     "ion_surrogate_get_stats": (_I, [_VP, ctypes.POINTER(_SurStats)]),
     "ion_surrogate_param_count": (_I, [_VP, ctypes.POINTER(_U64)]),
     "ion_surrogate_read_params": (_I, [_VP, _I, _VP, _U64]),
@@ -537,6 +538,13 @@ def hbm_copy_probe(nbytes=1 << 30, reps=10):
     ms, gbs = ctypes.c_double(), ctypes.c_double()
     _check(load_library().ion_hbm_copy_probe(int(nbytes), int(reps), ctypes.byref(ms), ctypes.byref(gbs)))
     return ms.value, gbs.value
+
+
+def psi_math_check():
+    """(sqrt mismatches, division mismatches, division samples) of the static-field kernel's operators."""
+    a, b, c = _U64(), _U64(), _U64()
+    _check(load_library().ion_psi_math_check(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+    return a.value, b.value, c.value
 
 
 def device_codec(float_type, direction, data):

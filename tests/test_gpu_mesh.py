@@ -140,3 +140,24 @@ def test_deeva_thruster_at_reduced_resolution():
     meshes = [(os.path.join(STL, f), o, m, v) for f, o, m, v in meshes]
     lbm, flags = run_both((32, 64, 32), meshes, lbm_length=128.0 * s)
     assert np.count_nonzero(flags & 0x10) > 1000
+
+
+def test_psi_operators_match_the_general_ones():
+    """The static-field kernel's specialised square root (every integer in
+    [1, 2^24)) and division (sampled over its operand range) give the same
+    bits as the compiler's general correctly rounded operators."""
+    from ionsolver_amd.lbm import psi_math_check
+    sq, dv, samples = psi_math_check()
+    assert samples >= 1 << 28
+    assert sq == 0 and dv == 0
+
+
+@pytest.mark.parametrize("m", [(3.0e-13, 0.0, -2.0e-13), (1.0e13, 2.0e13, 0.0)])
+def test_precompute_b_outside_the_specialised_range(tmp_path, m):
+    """Magnetisations below 2^-40 or above 2^40 (lattice units) take the
+    general operators; the result is still exact against the oracle."""
+    shape = (18, 16, 20)
+    sphere, _, _ = icosphere((9.3, 7.6, 10.2), 4.7, subdiv=2)
+    meshes = [(write_stl(tmp_path / "s.stl", sphere), (0.0, 0.0, 0.0), ModelType.Magnet, m)]
+    lbm, flags = run_both(shape, meshes)
+    assert np.count_nonzero(flags & 0x10) > 100
