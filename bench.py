@@ -384,6 +384,18 @@ def main():
                              "note": "best of 10 nontemporal 16-byte copies of 1 GiB (read + write)"}
     kernels["stream_collide"]["frac_of_copy_probe"] = round(kernels["stream_collide"]["GB/s"] / copy_gbs, 4) \
         if kernels["stream_collide"]["GB/s"] else None
+    # the same access shape as stream_collide (one work-item per cell, 19 or 45
+    # SoA streams of the storage element) without its arithmetic
+    elem = 4 if int(cfg.float_type) == 2 else 2
+    streams = 45 if cfg.ext_magneto_hydro else 19
+    cells = min(N, 1 << 24)
+    _, pat_gbs = I.hbm_stream_probe(elem, streams, cells, 10)
+    out["pattern_probe"] = {"GB/s": round(pat_gbs, 1), "frac_of_peak": round(pat_gbs / HBM_PEAK_GBS, 4),
+                            "streams": streams, "bytes_per_element": elem, "cells": cells,
+                            "note": "one work-item per cell loading its SoA stream elements then storing them to "
+                                    "a second buffer, nontemporal, best of 10 (read + write)"}
+    kernels["stream_collide"]["frac_of_pattern_probe"] = round(kernels["stream_collide"]["GB/s"] / pat_gbs, 4) \
+        if kernels["stream_collide"]["GB/s"] else None
     if thruster:
         out["surrogate"] = {k2: v for k2, v in lbm.surrogate_stats().items()}
         out["surrogate"]["note"] = ("field steps of the whole run including warm-up; the first is the direct "

@@ -261,6 +261,13 @@ int ion_surrogate_write_params(ion_handle* h, const float* src, uint64_t n);
  * scales the FETCH_SIZE / WRITE_SIZE counters of profiles.
  * ------------------------------------------------------------------------ */
 int ion_hbm_copy_probe(uint64_t bytes, uint32_t reps, double* best_ms, double* gb_per_s);
+/* Measurement helper (no reference counterpart): stream_collide's access
+ * shape without its arithmetic. One work-item per cell reads `streams`
+ * (19 or 45) structure-of-arrays elements of `elem_bytes` (2 or 4), then
+ * writes them to a second buffer at the same positions, nontemporal; best of
+ * reps + 1 runs, rate in GB/s (read + write). */
+int ion_hbm_stream_probe(uint32_t elem_bytes, uint32_t streams, uint32_t cells, uint32_t reps, double* best_ms,
+                         double* gb_per_s);
 
 /* ------------------------------------------------------------------------
  * Meshes and static fields (src/mesh.rs, src/lbm/mod.rs:284-314).

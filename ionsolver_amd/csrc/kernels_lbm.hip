@@ -182,8 +182,11 @@ __global__ void __launch_bounds__(256) k_initialize(LbmArgs a) {
 // (lod_bx consecutive cells of one row); the per-cell contributions are
 // summed per segment by the pairwise tree in LDS and written to lod_seg.
 // ---------------------------------------------------------------------------
+// MHD variants with 32-bit arithmetic are held to 128 VGPRs (4 wavefronts per
+// SIMD): the 16-bit formats otherwise use a few more and drop to 3.
 template <int Q, class S, bool MHD, bool TRT, bool LOD>
-__global__ void __launch_bounds__(MHD ? 512 : 256) k_stream_collide(LbmArgs a) {
+__global__ void __launch_bounds__(MHD ? 512 : 256)
+    __attribute__((amdgpu_waves_per_eu(MHD && sizeof(typename S::R) == 4 ? 4 : 1))) k_stream_collide(LbmArgs a) {
     typedef typename S::R R;
     typedef typename V4<R>::T V4;
     (void)sizeof(V4);

@@ -16,6 +16,25 @@ __global__ void __launch_bounds__(256) k_copy_probe(const v4f* __restrict__ a, v
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         __builtin_nontemporal_store(__builtin_nontemporal_load(&a[i]), &b[i]);
 }
+
+// stream_collide's access shape without its arithmetic: one work-item per
+// cell reads S structure-of-arrays streams a[k * n + cell] (all loads first)
+// and writes them to b[k * n + cell], nontemporal, elements of type T
+template <int S, class T>
+__global__ void __launch_bounds__(256) k_stream_probe(const T* __restrict__ a, T* __restrict__ b, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T v[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) v[k] = __builtin_nontemporal_load(&a[(uint64_t)k * n + i]);
+#pragma unroll
+    for (int k = 0; k < S; k++) __builtin_nontemporal_store(v[k], &b[(uint64_t)k * n + i]);
+}
+
+template <int S, class T>
+void launch_stream_probe(const void* a, void* b, uint32_t n) {
+    hipLaunchKernelGGL((k_stream_probe<S, T>), dim3((n + 255) / 256), dim3(256), 0, 0, (const T*)a, (T*)b, n);
+}
 }  // namespace
 
 extern "C" __attribute__((visibility("hidden"))) int ion_internal_set_error(int code, const char* msg);
@@ -54,5 +73,51 @@ extern "C" int ion_hbm_copy_probe(uint64_t bytes, uint32_t reps, double* best_ms
     }
     *best_ms = best;
     *gb_per_s = 2.0 * (double)(n * 16) / (best * 1e-3) / 1e9;
+    return ION_OK;
+}
+
+extern "C" int ion_hbm_stream_probe(uint32_t elem_bytes, uint32_t streams, uint32_t cells, uint32_t reps,
+                                    double* best_ms, double* gb_per_s) {
+    if (!best_ms || !gb_per_s || cells == 0 || reps == 0 || (elem_bytes != 2 && elem_bytes != 4) ||
+        (streams != 19 && streams != 45)) {
+        return ion_internal_set_error(ION_ERR_INVALID,
+                                      "ion_hbm_stream_probe: elem_bytes 2 or 4, streams 19 or 45, cells and reps >= 1");
+    }
+    const uint64_t bytes = (uint64_t)elem_bytes * streams * cells;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipMemset(a, 0, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float best = 1e30f;
+    for (uint32_t r = 0; e == hipSuccess && r < reps + 1; r++) {
+        e = hipEventRecord(e0, 0);
+        if (e == hipSuccess) {
+            if (elem_bytes == 2) {
+                if (streams == 19) launch_stream_probe<19, uint16_t>(a, b, cells);
+                else launch_stream_probe<45, uint16_t>(a, b, cells);
+            } else {
+                if (streams == 19) launch_stream_probe<19, uint32_t>(a, b, cells);
+                else launch_stream_probe<45, uint32_t>(a, b, cells);
+            }
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, 0);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && r > 0 && ms < best) best = ms;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) {
+        return ion_internal_set_error(ION_ERR_DEVICE, (std::string("ion_hbm_stream_probe: ") + hipGetErrorString(e)).c_str());
+    }
+    *best_ms = best;
+    *gb_per_s = 2.0 * (double)bytes / (best * 1e-3) / 1e9;
     return ION_OK;
 }

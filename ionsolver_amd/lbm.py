@@ -177,6 +177,7 @@ ABI = {
     "ion_set_field_solver": (_I, [_VP, _I, ctypes.c_uint32, _D]),
     "ion_surrogate_set_learning_rate": (_I, [_VP, ctypes.c_float, ctypes.c_float]),
     "ion_hbm_copy_probe": (_I, [_U64, ctypes.c_uint32, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "ion_hbm_stream_probe": (_I, [ctypes.c_uint32] * 4 + [ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "ion_import_mesh": (_I, [_VP, ctypes.c_char_p] + [_F] * 7 + [ctypes.POINTER(ctypes.c_uint32)]),
     "ion_import_mesh_reposition": (_I, [_VP, ctypes.c_char_p] + [_F] * 7 + [ctypes.POINTER(ctypes.c_uint32)]),
     "ion_mesh_get": (_I, [_VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), _VP, _VP]),
@@ -537,6 +538,14 @@ def hbm_copy_probe(nbytes=1 << 30, reps=10):
     """Best (ms, GB/s read + write) of a plain 16-byte streaming copy of nbytes."""
     ms, gbs = ctypes.c_double(), ctypes.c_double()
     _check(load_library().ion_hbm_copy_probe(int(nbytes), int(reps), ctypes.byref(ms), ctypes.byref(gbs)))
+    return ms.value, gbs.value
+
+
+def hbm_stream_probe(elem_bytes, streams, cells, reps=10):
+    """Best (ms, GB/s read + write) of stream_collide's access shape without arithmetic."""
+    ms, gbs = ctypes.c_double(), ctypes.c_double()
+    _check(load_library().ion_hbm_stream_probe(int(elem_bytes), int(streams), int(cells), int(reps),
+                                               ctypes.byref(ms), ctypes.byref(gbs)))
     return ms.value, gbs.value
 
 
