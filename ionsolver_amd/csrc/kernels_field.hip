@@ -284,16 +284,18 @@ __global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b_f64(FieldArgs a) {
 }
 
 // FAST field arithmetic on LOD-aligned tiles. A block covers
-// (4*TX) x TY x TZ cells inside one top-level LOD block, so the own-LOD
-// skip is the same for every cell of the block. Each thread owns 4
-// consecutive cells of one row: the source terms that depend only on y and z
-// (r_y, r_z, r_y^2 + r_z^2 and the x component of v x r) are computed once per
-// source and shared by the 4 cells.
-constexpr int ROW = 4;
-
+// (ROW*TX) x TY x TZ cells inside one top-level LOD block, so the own-LOD
+// skip is the same for every cell of the block: the thread that stages the
+// own source into LDS records its tile slot, and the source loop runs in two
+// branch-free segments around it. Each thread owns ROW consecutive cells of
+// one row: the source terms that depend only on y and z (r_y, r_z,
+// r_y^2 + r_z^2 and the x component of v x r) are computed once per source
+// and shared by the ROW cells.
+template <int ROW>
 __global__ void __launch_bounds__(256) k_update_e_b_fast_tiled(FieldArgs a, uint32_t TX, uint32_t TY, uint32_t TZ) {
     __shared__ float4 s_pq[FIELD_TILE];
     __shared__ float4 s_vd[FIELD_TILE];
+    __shared__ uint32_t s_own;
     const uint32_t NX = a.nx, NY = a.ny;
     const uint32_t tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);
     const uint32_t tiles_x = NX / (ROW * TX), tiles_y = NY / TY;
@@ -309,36 +311,44 @@ __global__ void __launch_bounds__(256) k_update_e_b_fast_tiled(FieldArgs a, uint
         px[k] = (float)(x0 + k);
         ex[k] = ey[k] = ez[k] = bxa[k] = bya[k] = bza[k] = 0.0f;
     }
+    auto source = [&](uint32_t s) {
+        const float4 vd = s_vd[s];
+        const float4 pq = s_pq[s];
+        const float ry = py - pq.y, rz = pz - pq.z;
+        const float syz = __builtin_fmaf(ry, ry, rz * rz);
+        const float cx = __builtin_fmaf(vd.y, rz, -(vd.z * ry));  // (v x r).x
+        const float m1 = vd.x * rz, m2 = vd.x * ry;
+#pragma unroll
+        for (int k = 0; k < ROW; k++) {
+            const float rx = px[k] - pq.x;
+            const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, syz));
+            const float t = pq.w * (inv * inv * inv);
+            ex[k] = __builtin_fmaf(t, rx, ex[k]);
+            ey[k] = __builtin_fmaf(t, ry, ey[k]);
+            ez[k] = __builtin_fmaf(t, rz, ez[k]);
+            bxa[k] = __builtin_fmaf(t, cx, bxa[k]);
+            bya[k] = __builtin_fmaf(t, __builtin_fmaf(vd.z, rx, -m1), bya[k]);  // (v x r).y
+            bza[k] = __builtin_fmaf(t, __builtin_fmaf(-vd.y, rx, m2), bza[k]);  // (v x r).z
+        }
+    };
     const uint32_t count = *a.src_count;
     for (uint32_t base = 0; base < count; base += FIELD_TILE) {
         const uint32_t m = min(FIELD_TILE, count - base);
         __syncthreads();
+        if (threadIdx.x == 0) s_own = m;
+        __syncthreads();
         for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+            const float4 vd = a.src_vd[base + i];
             s_pq[i] = a.src_pq[base + i];
-            s_vd[i] = a.src_vd[base + i];
+            s_vd[i] = vd;
+            if (__float_as_uint(vd.w) == own) s_own = i;  // at most one source is the own LOD
         }
         __syncthreads();
-        for (uint32_t s = 0; s < m; s++) {
-            const float4 vd = s_vd[s];
-            if (__float_as_uint(vd.w) == own) continue;  // uniform over the block
-            const float4 pq = s_pq[s];
-            const float ry = py - pq.y, rz = pz - pq.z;
-            const float syz = __builtin_fmaf(ry, ry, rz * rz);
-            const float cx = __builtin_fmaf(vd.y, rz, -(vd.z * ry));  // (v x r).x
-            const float m1 = vd.x * rz, m2 = vd.x * ry;
-#pragma unroll
-            for (int k = 0; k < ROW; k++) {
-                const float rx = px[k] - pq.x;
-                const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, syz));
-                const float t = pq.w * (inv * inv * inv);
-                ex[k] = __builtin_fmaf(t, rx, ex[k]);
-                ey[k] = __builtin_fmaf(t, ry, ey[k]);
-                ez[k] = __builtin_fmaf(t, rz, ez[k]);
-                bxa[k] = __builtin_fmaf(t, cx, bxa[k]);
-                bya[k] = __builtin_fmaf(t, __builtin_fmaf(vd.z, rx, -m1), bya[k]);  // (v x r).y
-                bza[k] = __builtin_fmaf(t, __builtin_fmaf(-vd.y, rx, m2), bza[k]);  // (v x r).z
-            }
-        }
+        const uint32_t skip = s_own;
+#pragma unroll 2
+        for (uint32_t s = 0; s < skip; s++) source(s);
+#pragma unroll 2
+        for (uint32_t s = skip + 1; s < m; s++) source(s);
     }
     const uint64_t N = a.N;
 #pragma unroll
@@ -426,15 +436,16 @@ hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s) {
         // LOD-aligned tiles: (ROW*TX) | lod block x, TY | lod block y, TZ | lod block z
         const uint32_t nd = 1u << a.D;
         const uint32_t lbx = a.nx / nd, lby = a.ny / nd, lbz = a.nz / nd;
-        if (lbx % ROW == 0) {
-            const uint32_t TX = pow2_div(lbx / ROW, 16);
+        for (uint32_t row : {8u, 4u}) {
+            if (lbx % row) continue;
+            const uint32_t TX = pow2_div(lbx / row, 16);
             const uint32_t TY = pow2_div(lby, 256 / TX);
             const uint32_t TZ = pow2_div(lbz, 256 / (TX * TY));
-            if (TX * TY * TZ >= 64) {
-                const uint32_t blocks = (a.nx / (ROW * TX)) * (a.ny / TY) * (a.nz / TZ);
-                hipLaunchKernelGGL(k_update_e_b_fast_tiled, dim3(blocks), dim3(TX * TY * TZ), 0, s, a, TX, TY, TZ);
-                return hipGetLastError();
-            }
+            if (TX * TY * TZ < 64) continue;
+            const uint32_t blocks = (a.nx / (row * TX)) * (a.ny / TY) * (a.nz / TZ);
+            if (row == 8) hipLaunchKernelGGL(k_update_e_b_fast_tiled<8>, dim3(blocks), dim3(TX * TY * TZ), 0, s, a, TX, TY, TZ);
+            else hipLaunchKernelGGL(k_update_e_b_fast_tiled<4>, dim3(blocks), dim3(TX * TY * TZ), 0, s, a, TX, TY, TZ);
+            return hipGetLastError();
         }
     }
     const int cpt = fast ? CPT_FAST : CPT_EXACT;
