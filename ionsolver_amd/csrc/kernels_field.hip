@@ -350,27 +350,47 @@ __global__ void __launch_bounds__(256) k_update_e_b_fast_tiled(FieldArgs a, uint
 #pragma unroll 2
         for (uint32_t s = skip + 1; s < m; s++) source(s);
     }
+    // Epilogue through LDS, one component at a time: each work-item's ROW
+    // consecutive cells go to LDS in block order, then the block writes whole
+    // row segments (ROW*TX cells) with consecutive lanes on consecutive cells,
+    // so that every store (and E_stat / B_stat load) is contiguous.
     const uint64_t N = a.N;
+    const uint32_t W = ROW * TX, cells = ROW * blockDim.x;
+    float* const lds = (float*)s_pq;  // cells floats fit in the source tile
+    const uint32_t own0 = x0 - bx * W + W * (ty + TY * tz);  // first own cell in block order
+    uint32_t live = 0;  // bit j: cell threadIdx.x + j * blockDim.x is not solid
+    uint64_t gn[ROW];
 #pragma unroll
-    for (int k = 0; k < ROW; k++) {
-        const uint32_t n = (x0 + k) + (y + z * NY) * NX;
-        if ((a.flags[n] & 0x1F) == 0x01) continue;
-        if (a.raw != nullptr) {
-            float* r = a.raw;
-            r[n] = ex[k];
-            r[N + n] = ey[k];
-            r[2 * N + n] = ez[k];
-            r[3 * N + n] = bxa[k];
-            r[4 * N + n] = bya[k];
-            r[5 * N + n] = bza[k];
-            continue;
+    for (int j = 0; j < ROW; j++) {
+        const uint32_t i = threadIdx.x + j * blockDim.x;
+        const uint32_t lx = i % W, ly = (i / W) % TY, lz = i / (W * TY);
+        gn[j] = (uint64_t)(bx * W + lx) + ((uint64_t)(by * TY + ly) + (uint64_t)(bz * TZ + lz) * NY) * NX;
+        if ((a.flags[gn[j]] & 0x1F) != 0x01) live |= 1u << j;
+    }
+    (void)cells;
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ROW; k++) {
+            const float v = c == 0 ? ex[k] : c == 1 ? ey[k] : c == 2 ? ez[k] : c == 3 ? bxa[k] : c == 4 ? bya[k] : bza[k];
+            lds[own0 + k] = v;
         }
-        a.e_dyn[n] = a.e_stat[n] + a.ke * ex[k];
-        a.e_dyn[N + n] = a.e_stat[N + n] + a.ke * ey[k];
-        a.e_dyn[2 * N + n] = a.e_stat[2 * N + n] + a.ke * ez[k];
-        a.b_dyn[n] = a.b_stat[n] + a.kmu * bxa[k];
-        a.b_dyn[N + n] = a.b_stat[N + n] + a.kmu * bya[k];
-        a.b_dyn[2 * N + n] = a.b_stat[2 * N + n] + a.kmu * bza[k];
+        __syncthreads();
+        const uint64_t off = (uint64_t)(c % 3) * N;
+#pragma unroll
+        for (int j = 0; j < ROW; j++) {
+            if (!(live >> j & 1u)) continue;
+            const float v = lds[threadIdx.x + j * blockDim.x];
+            const uint64_t n = gn[j];
+            if (a.raw != nullptr) {
+                a.raw[(uint64_t)c * N + n] = v;
+            } else if (c < 3) {
+                a.e_dyn[off + n] = a.e_stat[off + n] + a.ke * v;
+            } else {
+                a.b_dyn[off + n] = a.b_stat[off + n] + a.kmu * v;
+            }
+        }
     }
 }
 
