@@ -378,21 +378,24 @@ int validate(const ion_config& c) {
     return ION_OK;
 }
 
-// stream_collide block: a contiguous range of whole LOD row segments (at
-// most 512 cells) when the level-1 LOD tree is fused in, else 256 cells.
+// stream_collide block: a contiguous range of whole LOD row segments when
+// the level-1 LOD tree is fused in (whole rows up to 256 cells, else 256
+// cells of a row where the segments allow, at most 512), else 256 cells.
 int choose_block(ion_handle* h) {
     if (!h->lod) { h->sc_block = 256; return ION_OK; }
     const uint32_t NX = h->cfg.n_x, bx = h->lod_bx;
     const uint64_t rows = (uint64_t)h->cfg.n_y * h->cfg.n_z;
-    if (NX <= 512) {
+    if (NX <= 256) {
         uint32_t ty = 1;
         for (uint32_t t = 2; (uint64_t)NX * t <= 256; t++)
             if (rows % t == 0) ty = t;
         h->sc_block = NX * ty;
         return ION_OK;
     }
-    for (uint32_t b = 512; b >= bx && b > 0; b--)
-        if (NX % b == 0 && b % bx == 0) { h->sc_block = b; return ION_OK; }
+    // wider rows: part of a row, whole LOD segments, preferably 256 cells
+    for (uint32_t cap : {256u, 512u})
+        for (uint32_t b = cap; b >= bx && b > 0; b--)
+            if (NX % b == 0 && b % bx == 0) { h->sc_block = b; return ION_OK; }
     return fail(ION_ERR_UNSUPPORTED, "LOD block wider than 512 cells along x");
 }
 

@@ -43,6 +43,16 @@ def test_mhd_d3q19(float_type, lod_depth):
     assert compare_all(gpu, orc, info) == []
 
 
+@pytest.mark.parametrize("nx,ny,nz,D,ft", [(512, 4, 4, 2, 1), (384, 8, 8, 2, 2), (1024, 2, 4, 1, 0)])
+def test_mhd_rows_wider_than_a_block(nx, ny, nz, D, ft):
+    """Rows longer than 256 cells: stream_collide blocks hold part of a row
+    (whole LOD segments), every buffer exact."""
+    rng = np.random.default_rng(500 + nx + D)
+    gpu, orc, info = make_pair(rng, nx=nx, ny=ny, nz=nz, mhd=True, lod_depth=D, float_type=ft)
+    run_both(gpu, orc, 3)
+    assert compare_all(gpu, orc, info) == []
+
+
 def test_mhd_lod_depth0_and_d3q15():
     rng = np.random.default_rng(400)
     gpu, orc, info = make_pair(rng, nx=8, ny=8, nz=8, mhd=True, lod_depth=0, float_type=2)
