@@ -319,7 +319,15 @@ def main():
     lbm.set_timing(True)
     lbm.synchronize()
     t0 = time.perf_counter()
-    lbm.run(args.steps)  # enqueues all steps, synchronises at the end
+    done, shown = 0, t0
+    while done < args.steps:  # in chunks of up to 25 steps, a progress line at least every 30 s
+        k_run = min(25, args.steps - done)
+        lbm.run(k_run)  # enqueues the steps, synchronises at the end
+        done += k_run
+        now = time.perf_counter()
+        if now - shown > 30.0:
+            print(f"step {done}/{args.steps}: {now - t0:.1f} s", file=sys.stderr, flush=True)
+            shown = now
     dt = time.perf_counter() - t0
     tm = lbm.get_timing()
     lbm.set_timing(False)
