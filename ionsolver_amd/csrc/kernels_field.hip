@@ -9,6 +9,7 @@
 // in ascending LOD index as the reference loop does.
 #include <hip/hip_runtime.h>
 
+#include "cr_math.hpp"
 #include "kernels.hpp"
 
 namespace ion {
@@ -283,16 +284,20 @@ __global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b_f64(FieldArgs a) {
     bd[2 * N + n] = bs[2 * N + n] + kmu * bz;
 }
 
-// FAST field arithmetic on LOD-aligned tiles. A block covers
+// The field update on LOD-aligned tiles, FAST or EXACT arithmetic. A block covers
 // (ROW*TX) x TY x TZ cells inside one top-level LOD block, so the own-LOD
 // skip is the same for every cell of the block: the thread that stages the
 // own source into LDS records its tile slot, and the source loop runs in two
 // branch-free segments around it. Each thread owns ROW consecutive cells of
 // one row: the source terms that depend only on y and z (r_y, r_z,
 // r_y^2 + r_z^2 and the x component of v x r) are computed once per source
-// and shared by the ROW cells.
-template <int ROW>
-__global__ void __launch_bounds__(256) k_update_e_b_fast_tiled(FieldArgs a, uint32_t TX, uint32_t TY, uint32_t TZ) {
+// and shared by the ROW cells. EXACT keeps the reference's arithmetic per
+// interaction (only r_y^2 and r_z^2 are shared, the sum order is unchanged)
+// with the specialised correctly rounded operators of cr_math.hpp: |r|^2 is a
+// multiple of 1/4 in [1/4, 2^24) (cell minus block centre), |r|^3 lies in
+// [1/8, 2^36] and the dividends r_i are 0 or in [1/2, 2^11].
+template <int ROW, bool FAST>
+__global__ void __launch_bounds__(256) k_update_e_b_tiled(FieldArgs a, uint32_t TX, uint32_t TY, uint32_t TZ) {
     __shared__ float4 s_pq[FIELD_TILE];
     __shared__ float4 s_vd[FIELD_TILE];
     __shared__ uint32_t s_own;
@@ -315,20 +320,41 @@ __global__ void __launch_bounds__(256) k_update_e_b_fast_tiled(FieldArgs a, uint
         const float4 vd = s_vd[s];
         const float4 pq = s_pq[s];
         const float ry = py - pq.y, rz = pz - pq.z;
-        const float syz = __builtin_fmaf(ry, ry, rz * rz);
-        const float cx = __builtin_fmaf(vd.y, rz, -(vd.z * ry));  // (v x r).x
-        const float m1 = vd.x * rz, m2 = vd.x * ry;
+        if constexpr (FAST) {
+            const float syz = __builtin_fmaf(ry, ry, rz * rz);
+            const float cx = __builtin_fmaf(vd.y, rz, -(vd.z * ry));  // (v x r).x
+            const float m1 = vd.x * rz, m2 = vd.x * ry;
 #pragma unroll
-        for (int k = 0; k < ROW; k++) {
-            const float rx = px[k] - pq.x;
-            const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, syz));
-            const float t = pq.w * (inv * inv * inv);
-            ex[k] = __builtin_fmaf(t, rx, ex[k]);
-            ey[k] = __builtin_fmaf(t, ry, ey[k]);
-            ez[k] = __builtin_fmaf(t, rz, ez[k]);
-            bxa[k] = __builtin_fmaf(t, cx, bxa[k]);
-            bya[k] = __builtin_fmaf(t, __builtin_fmaf(vd.z, rx, -m1), bya[k]);  // (v x r).y
-            bza[k] = __builtin_fmaf(t, __builtin_fmaf(-vd.y, rx, m2), bza[k]);  // (v x r).z
+            for (int k = 0; k < ROW; k++) {
+                const float rx = px[k] - pq.x;
+                const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, syz));
+                const float t = pq.w * (inv * inv * inv);
+                ex[k] = __builtin_fmaf(t, rx, ex[k]);
+                ey[k] = __builtin_fmaf(t, ry, ey[k]);
+                ez[k] = __builtin_fmaf(t, rz, ez[k]);
+                bxa[k] = __builtin_fmaf(t, cx, bxa[k]);
+                bya[k] = __builtin_fmaf(t, __builtin_fmaf(vd.z, rx, -m1), bya[k]);  // (v x r).y
+                bza[k] = __builtin_fmaf(t, __builtin_fmaf(-vd.y, rx, m2), bza[k]);  // (v x r).z
+            }
+        } else {
+            // interact_exact with r_y^2, r_z^2 hoisted: pre = r / |r|^3,
+            // e += q pre, b += q (v x pre), every operation rounded on its own
+            const float ry2 = ry * ry, rz2 = rz * rz;
+            const float q = pq.w;
+#pragma unroll
+            for (int k = 0; k < ROW; k++) {
+                const float rx = px[k] - pq.x;
+                const float sm = sqrt_cr((rx * rx + ry2) + rz2);
+                const float cbm = (sm * sm) * sm;
+                const float prx = div_cr(rx, cbm), pry = div_cr(ry, cbm), prz = div_cr(rz, cbm);
+                ex[k] = ex[k] + q * prx;
+                ey[k] = ey[k] + q * pry;
+                ez[k] = ez[k] + q * prz;
+                const float crx = vd.y * prz - vd.z * pry, cry = vd.z * prx - vd.x * prz, crz = vd.x * pry - vd.y * prx;
+                bxa[k] = bxa[k] + q * crx;
+                bya[k] = bya[k] + q * cry;
+                bza[k] = bza[k] + q * crz;
+            }
         }
     };
     const uint32_t count = *a.src_count;
@@ -452,7 +478,7 @@ hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s) {
                            dim3(FIELD_BLOCK), 0, s, a);
         return hipGetLastError();
     }
-    if (fast && !a.near && a.D > 0) {
+    if (!a.near && a.D > 0) {
         // LOD-aligned tiles: (ROW*TX) | lod block x, TY | lod block y, TZ | lod block z
         const uint32_t nd = 1u << a.D;
         const uint32_t lbx = a.nx / nd, lby = a.ny / nd, lbz = a.nz / nd;
@@ -463,8 +489,14 @@ hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s) {
             const uint32_t TZ = pow2_div(lbz, 256 / (TX * TY));
             if (TX * TY * TZ < 64) continue;
             const uint32_t blocks = (a.nx / (row * TX)) * (a.ny / TY) * (a.nz / TZ);
-            if (row == 8) hipLaunchKernelGGL(k_update_e_b_fast_tiled<8>, dim3(blocks), dim3(TX * TY * TZ), 0, s, a, TX, TY, TZ);
-            else hipLaunchKernelGGL(k_update_e_b_fast_tiled<4>, dim3(blocks), dim3(TX * TY * TZ), 0, s, a, TX, TY, TZ);
+            const dim3 g(blocks), bl(TX * TY * TZ);
+            if (fast) {
+                if (row == 8) hipLaunchKernelGGL((k_update_e_b_tiled<8, true>), g, bl, 0, s, a, TX, TY, TZ);
+                else hipLaunchKernelGGL((k_update_e_b_tiled<4, true>), g, bl, 0, s, a, TX, TY, TZ);
+            } else {
+                if (row == 8) hipLaunchKernelGGL((k_update_e_b_tiled<8, false>), g, bl, 0, s, a, TX, TY, TZ);
+                else hipLaunchKernelGGL((k_update_e_b_tiled<4, false>), g, bl, 0, s, a, TX, TY, TZ);
+            }
             return hipGetLastError();
         }
     }

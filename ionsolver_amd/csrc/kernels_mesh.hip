@@ -11,6 +11,7 @@
 // reference. All arithmetic is the reference's, unfused and correctly
 // rounded (the library is built with -ffp-contract=off and correctly rounded
 // division and square root).
+#include "cr_math.hpp"
 #include "mesh.hpp"
 
 namespace ion {
@@ -234,46 +235,15 @@ __global__ __launch_bounds__(SEL_BLOCK) void k_select_scatter(int mode, const ui
 // ---------------------------------------------------------------------------
 // psi_from_mesh (sim_kernels.cl:1234-1251) over compacted magnet sources.
 //
-// The correctly rounded square root and division the compiler emits
-// (-fhip-fp32-correctly-rounded-divide-sqrt) are, for operands in normal
-// range, v_sqrt_f32 followed by a one-ulp correction either way, and
-// v_rcp_f32 followed by Newton and two residual corrections, wrapped in
-// range scaling (v_div_scale / v_div_fmas) and special-value handling
-// (v_div_fixup, class tests). k_psi's operands stay where the wrapping is the
-// identity: |r|^2 is an exact integer in [1, 2^24) (cell offsets), |r|^3 lies
-// in [1, 2^36], and with every magnetisation component 0 or within
-// [2^-40, 2^40] the dividend r.M is 0 or within [2^-63, 2^54], the quotient
-// normal. So the core sequences alone give the same bits; the scatter pass
-// checks the magnetisation range and k_psi falls back to the general
-// operators otherwise. run_psi_math_check compares the square root over every
-// integer in [1, 2^24) and the division over a large sample.
+// The operands keep the specialised correctly rounded operators of
+// cr_math.hpp exact: |r|^2 is an exact integer in [1, 2^24) (cell offsets),
+// |r|^3 lies in [1, 2^36], and with every magnetisation component 0 or
+// within [2^-40, 2^40] the dividend r.M is 0 or within [2^-63, 2^54], the
+// quotient normal. The scatter pass checks the magnetisation range and k_psi
+// falls back to the general operators otherwise.
 // ---------------------------------------------------------------------------
 constexpr uint32_t FLD_BLOCK = 256;
 constexpr uint32_t FLD_TILE = 256;
-
-// correctly rounded sqrt of x in [1, 2^24)
-__device__ __forceinline__ float sqrt_cr(float x) {
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float s_dn = __uint_as_float(__float_as_uint(s) - 1u);
-    const float s_up = __uint_as_float(__float_as_uint(s) + 1u);
-    const float r_dn = __builtin_fmaf(-s_dn, s, x);
-    const float r_up = __builtin_fmaf(-s_up, s, x);
-    const float t = r_dn <= 0.0f ? s_dn : s;
-    return r_up > 0.0f ? s_up : t;
-}
-
-// correctly rounded a / b for b in [1, 2^36] and a == 0 or |a| in [2^-63, 2^54]
-// (a == 0 may give a zero of the other sign)
-__device__ __forceinline__ float div_cr(float a, float b) {
-    const float y0 = __builtin_amdgcn_rcpf(b);
-    const float e = __builtin_fmaf(-b, y0, 1.0f);
-    const float y1 = __builtin_fmaf(e, y0, y0);
-    const float q0 = a * y1;
-    const float r0 = __builtin_fmaf(-b, q0, a);
-    const float q1 = __builtin_fmaf(r0, y1, q0);
-    const float r1 = __builtin_fmaf(-b, q1, a);
-    return __builtin_fmaf(r1, y1, q1);
-}
 
 template <bool SPECIAL>
 __device__ __forceinline__ float psi_tile(const float4* sp, const float4* sv, uint32_t m, float cx, float cy, float cz,
@@ -411,21 +381,24 @@ hipError_t launch_select_scatter(int mode, const uint8_t* flags, const float* m,
     return hipGetLastError();
 }
 
-__global__ void k_check_sqrt(unsigned long long* mismatch) {
+// every integer in [1, 2^24) and every quarter of one (|r|^2 of cell offsets
+// from cell or LOD-block centres)
+__global__ void k_check_sqrt(unsigned long long* mismatch, float scale) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x + 1u;  // 1 .. 2^24 - 1
     if (i >= (1u << 24)) return;
-    const float x = (float)i;
+    const float x = (float)i * scale;
     if (__float_as_uint(sqrt_cr(x)) != __float_as_uint(sqrtf(x))) atomicAdd(mismatch, 1ull);
 }
 
-// pseudo-random (a, b) over the specialised ranges, with b = l^3 of an integer l^2
+// pseudo-random (a, b) over the specialised ranges, with b = l^3 of an
+// integer or quarter-integer l^2
 __global__ void k_check_div(unsigned long long* mismatch, uint32_t seed) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t h = i * 0x9E3779B9u ^ seed;
     h ^= h >> 16; h *= 0x7FEB352Du; h ^= h >> 15; h *= 0x846CA68Bu; h ^= h >> 16;
     uint32_t g = h * 0x2C1B3C6Du + 0x297A2D39u;
     g ^= g >> 15; g *= 0x27D4EB2Du; g ^= g >> 13;
-    const float l2 = (float)(1u + (g % ((1u << 24) - 1u)));
+    const float l2 = (float)(1u + (g % ((1u << 24) - 1u))) * ((seed & 2u) ? 0.25f : 1.0f);
     const float l = sqrtf(l2);
     const float b = l * l * l;
     // a: random sign, exponent in [-63, 54], random mantissa
@@ -442,7 +415,8 @@ hipError_t run_psi_math_check(uint64_t* sqrt_mismatch, uint64_t* div_mismatch, u
     const uint32_t B = 256;
     const uint32_t rounds = 16, per = 1u << 24;
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_check_sqrt, dim3(((1u << 24) + B - 1) / B), dim3(B), 0, 0, d);
+        hipLaunchKernelGGL(k_check_sqrt, dim3(((1u << 24) + B - 1) / B), dim3(B), 0, 0, d, 1.0f);
+        hipLaunchKernelGGL(k_check_sqrt, dim3(((1u << 24) + B - 1) / B), dim3(B), 0, 0, d, 0.25f);
         for (uint32_t r = 0; r < rounds; r++) hipLaunchKernelGGL(k_check_div, dim3(per / B), dim3(B), 0, 0, d + 1, r * 7919u + 1u);
         e = hipGetLastError();
     }

@@ -53,6 +53,16 @@ def test_mhd_rows_wider_than_a_block(nx, ny, nz, D, ft):
     assert compare_all(gpu, orc, info) == []
 
 
+@pytest.mark.parametrize("shape,D", [((64, 64, 32), 2), ((96, 64, 64), 2)])
+def test_exact_field_on_lod_tiles(shape, D):
+    """Shapes whose LOD blocks the tiled field kernel covers (8 cells per
+    work-item), EXACT arithmetic: every buffer exact against the oracle."""
+    rng = np.random.default_rng(600 + shape[0])
+    gpu, orc, info = make_pair(rng, nx=shape[0], ny=shape[1], nz=shape[2], mhd=True, lod_depth=D, float_type=2)
+    run_both(gpu, orc, 2)
+    assert compare_all(gpu, orc, info) == []
+
+
 def test_mhd_lod_depth0_and_d3q15():
     rng = np.random.default_rng(400)
     gpu, orc, info = make_pair(rng, nx=8, ny=8, nz=8, mhd=True, lod_depth=0, float_type=2)
