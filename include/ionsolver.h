@@ -60,7 +60,8 @@ typedef struct ion_units {
  * from the Q buffer (set by the caller, like rho and u of TYPE_E cells),
  * the D3Q7 charge DDFs are reset to their equilibrium with that charge every
  * step (no collision), and Q is not written back. It is not part of the save
- * format (ion_load returns it as 0). */
+ * format: ion_load takes it from cfg_inout, like the other fields the file
+ * does not store. */
 typedef struct ion_config {
     int32_t velocity_set;
     int32_t relaxation_time;
