@@ -50,6 +50,8 @@ def bytes_per_cell(cfg):
     b = 2 * q * s + 1  # fi read + write, flags
     if cfg.ext_magneto_hydro:
         b += 2 * q * s + 2 * 7 * s + 24 + 4  # ei, fqi, E_dyn+B_dyn read, Q write
+    if cfg.ext_subgrid_ecr:
+        b += 2 * 7 * s + 12 + 4  # eti, E_var read, Et write (grad|B| reuses B_dyn through the caches)
     if cfg.graphics_config.graphics_active:
         b += 16  # rho, u written (UPDATE_FIELDS)
     return b
@@ -63,6 +65,8 @@ def workload(name):
         n = int(parts[0][3:])
         fmt = parts[1] if len(parts) > 1 else "fp32"
         cfg, arrays, blobs = scenarios.mhd_blobs(n=n, float_type=ft[fmt])
+        if "ecr" in parts[2:]:
+            scenarios.add_subgrid_ecr(cfg, arrays)
         return cfg, arrays, dict(blobs=len(blobs), seed=1234)
     if name.startswith("thruster"):
         n = int(name[8:].split("-")[0])

@@ -105,7 +105,11 @@ typedef enum ion_field {
      * update (E then B, 6 x N f32, before the KE / KMU scaling), kept once a
      * surrogate exists (the surrogate's training and judging reference) */
     ION_FIELD_FIELD_SUMS = 13,
-    ION_FIELD_COUNT = 14
+    /* ext_subgrid_ecr (src/lbm/domain.rs:200-211) */
+    ION_FIELD_E_VAR = 14,  /* oscillating electric field, 3 x N f32 */
+    ION_FIELD_ETI = 15,    /* electron temperature DDFs (D3Q7), 7 x N storage codes */
+    ION_FIELD_ET = 16,     /* electron temperature, N f32 */
+    ION_FIELD_COUNT = 17
 } ion_field;
 
 /* The f32 kernel constants the reference bakes in as #defines
@@ -120,6 +124,8 @@ typedef struct ion_kernel_constants {
     float kkge;  /* DEF_KKGE = electron mass/charge in lattice units */
     float wq;    /* DEF_WQ = 1/(2 k + 1/2), k = 1 */
     uint32_t n_lod;     /* DEF_NUM_LOD (single domain: = n_lod_own) */
+    float kkbme; /* DEF_KKBME = -1.5 k_B / m_e in lattice units (units.rs:187-189) */
+    float keabs; /* DEF_KEABS (units.rs:191-193) */
 } ion_kernel_constants;
 
 /* Field-update arithmetic (build-side option, no reference counterpart):
@@ -309,6 +315,9 @@ int ion_precompute_b(ion_handle* h);
 /* Lbm::precompute_E, src/lbm/mod.rs:302-314 (static_e_from_mesh): E_stat +=
  * Coulomb field of the charged cells. Synchronous. */
 int ion_precompute_e(ion_handle* h);
+/* Lbm::precompute_E_ECR, src/lbm/mod.rs:319-331: the same field into E_var
+ * (ext_subgrid_ecr). Synchronous. */
+int ion_precompute_e_ecr(ion_handle* h);
 /* Diagnostics (no reference counterpart): the specialised correctly rounded
  * square root and division of the precompute_B kernel against the general
  * ones on the current device: every integer in [1, 2^24) for the square root,

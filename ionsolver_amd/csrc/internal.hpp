@@ -20,6 +20,7 @@ struct HandleView {
     const ion_kernel_constants* kc;
     uint8_t* flags;
     float *e_stat, *b_stat, *e_dyn, *b_dyn;
+    float* e_var;  // SUBGRID_ECR only, else null
     MeshSet** meshes;
 };
 

@@ -34,6 +34,10 @@ struct LbmArgs {
     uint32_t odd;         // t % 2 (esoteric-pull layout)
     uint32_t eqb, vforce, ffield, upd;  // extension switches
     uint32_t qin;         // charge inlet: TYPE_E cells hold their charge (build extension)
+    void* eti;            // SUBGRID_ECR: electron temperature DDFs (D3Q7)
+    float* et;            // SUBGRID_ECR: electron temperature field
+    float* e_var;         // SUBGRID_ECR: oscillating electric field
+    float kkbme, keabs;   // DEF_KKBME, DEF_KEABS
     uint32_t lod_bx;      // LOD block side along x (cells)
     float fx, fy, fz;     // volume force
     float w;              // DEF_W
@@ -66,9 +70,10 @@ struct FieldArgs {
 };
 
 // kernels_lbm.hip
-hipError_t launch_initialize(int q, int store, bool mhd, const LbmArgs& a, hipStream_t s);
+// ecr: the SUBGRID_ECR variants (D3Q19, MHD, FP32/FP16S/FP16C)
+hipError_t launch_initialize(int q, int store, bool mhd, const LbmArgs& a, hipStream_t s, bool ecr = false);
 hipError_t launch_stream_collide(int q, int store, bool mhd, bool trt, bool lod, uint32_t block,
-                                 const LbmArgs& a, hipStream_t s);
+                                 const LbmArgs& a, hipStream_t s, bool ecr = false);
 hipError_t launch_update_fields(int q, int store, const LbmArgs& a, hipStream_t s);
 // Level-2 LOD tree: one block per top-level LOD.
 hipError_t launch_lod_reduce(const void* seg, void* qu_lod, bool f64, uint32_t nx, uint32_t ny, uint32_t nz,

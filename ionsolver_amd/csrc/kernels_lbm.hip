@@ -128,7 +128,7 @@ __device__ __forceinline__ void store_ddfs(typename S::T* __restrict__ buf, uint
 // ---------------------------------------------------------------------------
 // initialize, sim_kernels.cl:760-832 (writes the t = 1 layout)
 // ---------------------------------------------------------------------------
-template <int Q, class S, bool MHD>
+template <int Q, class S, bool MHD, bool ECR = false>
 __global__ void __launch_bounds__(256) k_initialize(LbmArgs a) {
     typedef typename S::R R;
     typedef typename V4<R>::T V4;
@@ -173,6 +173,10 @@ __global__ void __launch_bounds__(256) k_initialize(LbmArgs a) {
         }
         f_eq<Q>(R(0), ux, uy, uz, fe);
         store_ddfs<Q, Q, S>((typename S::T*)a.ei, N, n, b, 1u, fe);
+        if constexpr (ECR) {  // electron temperature DDFs, sim_kernels.cl:827-831
+            a_eq(R(a.et[n]), ux, uy, uz, qeq);
+            store_ddfs<19, 7, S>((typename S::T*)a.eti, N, n, b, 1u, qeq);
+        }
     }
 }
 
@@ -184,9 +188,9 @@ __global__ void __launch_bounds__(256) k_initialize(LbmArgs a) {
 // ---------------------------------------------------------------------------
 // MHD variants with 32-bit arithmetic are held to 128 VGPRs (4 wavefronts per
 // SIMD): the 16-bit formats otherwise use a few more and drop to 3.
-template <int Q, class S, bool MHD, bool TRT, bool LOD>
+template <int Q, class S, bool MHD, bool TRT, bool LOD, bool ECR = false>
 __global__ void __launch_bounds__(MHD ? 512 : 256)
-    __attribute__((amdgpu_waves_per_eu(MHD && sizeof(typename S::R) == 4 ? 4 : 1))) k_stream_collide(LbmArgs a) {
+    __attribute__((amdgpu_waves_per_eu(MHD && !ECR && sizeof(typename S::R) == 4 ? 4 : 1))) k_stream_collide(LbmArgs a) {
     typedef typename S::R R;
     typedef typename V4<R>::T V4;
     (void)sizeof(V4);
@@ -220,12 +224,13 @@ __global__ void __launch_bounds__(MHD ? 512 : 256)
         const bool is_e = a.eqb && bo == 0x02;
         const Nbr b = neighbours(n, a.nx, a.ny, a.nz);
         // every load of the cell first: fi, ei, fqi, E_dyn, B_dyn
-        T rf[Q], re[MHD ? Q : 1], rq[MHD ? 7 : 1];
+        T rf[Q], re[MHD ? Q : 1], rq[MHD ? 7 : 1], rt[ECR ? 7 : 1];
         R Bx = R(0), By = R(0), Bz = R(0), Ex = R(0), Ey = R(0), Ez = R(0);
         load_raw<Q, Q, S>((const T*)a.fi, N, n, b, a.odd, rf);
         if (MHD) {
             load_raw<Q, Q, S>((const T*)a.ei, N, n, b, a.odd, re);
             load_raw<19, 7, S>((const T*)a.fqi, N, n, b, a.odd, rq);
+            if constexpr (ECR) load_raw<19, 7, S>((const T*)a.eti, N, n, b, a.odd, rt);
             Bx = *at(Bd, n); By = *at(Bd + N, n); Bz = *at(Bd + 2 * N, n);
             Ex = *at(Ed, n); Ey = *at(Ed + N, n); Ez = *at(Ed + 2 * N, n);
         }
@@ -262,6 +267,51 @@ __global__ void __launch_bounds__(MHD ? 512 : 256)
 #pragma unroll
             for (int i = 0; i < 7; i++) rhon_q += qhn[i];
             rhon_q += R(1);
+            if constexpr (ECR) {
+                // SUBGRID_ECR, sim_kernels.cl:557-629 (the cyclotron frequency and
+                // relative absorption of :582-585 are computed there but unused)
+                R ethn[7];
+                decode<7, S>(rt, ethn);
+                R Etn = R(0);
+#pragma unroll
+                for (int i = 0; i < 7; i++) Etn += ethn[i];
+                Etn += R(1);
+                const R Evx = *at(a.e_var, n), Evy = *at(a.e_var + N, n), Evz = *at(a.e_var + 2 * N, n);
+                const R lb = sqrtf((Bx * Bx + By * By) + Bz * Bz);
+                const R sp = ((Evx * Bx + Evy * By) + Evz * Bz) / (lb * lb);
+                const R px = Evx - sp * Bx, py = Evy - sp * By, pz = Evz - sp * Bz;
+                const R env = sqrtf((px * px + py * py) + pz * pz);
+                Etn += R(a.keabs) / R(a.kkbme) * (rhon_e + R(0.00001f)) / KKGE * (env * env);
+                // grad_mag_v(n, B_dyn), sim_kernels.cl:453-461: mag(c + e) - mag(c - e) / 2
+                R g0 = R(0), g1 = R(0), g2 = R(0);
+                const uint32_t xx = b.x0, yy = b.y0 / a.nx, zz = b.z0 / (a.nx * a.ny);
+                if (!(xx == 0u || xx == a.nx - 1u || yy == 0u || yy == a.ny - 1u || zz == 0u || zz == a.nz - 1u)) {
+                    auto mag = [&](uint32_t i) {
+                        const R v0 = *at(Bd, i), v1 = *at(Bd + N, i), v2 = *at(Bd + 2 * N, i);
+                        return sqrtf((v0 * v0 + v1 * v1) + v2 * v2);
+                    };
+                    const uint32_t sy = a.nx, sz = a.nx * a.ny;
+                    g0 = mag(n + 1u) - mag(n - 1u) / R(2);
+                    g1 = mag(n + sy) - mag(n - sy) / R(2);
+                    g2 = mag(n + sz) - mag(n - sz) / R(2);
+                }
+                const R kt = R(a.kkbme) * Etn;
+                const R dx = kt * g0 / lb, dy = kt * g1 / lb, dz = kt * g2 / lb;
+                uxn_e += dx;
+                uyn_e += dy;
+                uzn_e += dz;
+                Etn -= sqrtf((dx * dx + dy * dy) + dz * dz) / R(a.kkbme);
+                R eteq[7];
+                a_eq(Etn, uxn_e, uyn_e, uzn_e, eteq);
+                if (a.upd) *at(a.et, n) = Etn;
+                const R wq = WQ;
+#pragma unroll
+                for (int i = 0; i < 7; i++) ethn[i] = fmaR(R(1) - wq, ethn[i], wq * eteq[i]);
+                store_ddfs<19, 7, S>((T*)a.eti, a.N_store, n, b, a.odd, ethn);
+                const R dq = R(0.0001f) * Etn;  // ionisation
+                rhon_e += dq;
+                rhon_q += dq;
+            }
             // charge inlet (build extension): a TYPE_E cell keeps the charge the
             // Q buffer holds and its charge DDFs are reset to that equilibrium
             const bool inlet = is_e && a.qin;
@@ -478,6 +528,23 @@ static hipError_t init_impl(bool mhd, const LbmArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// SUBGRID_ECR variants (D3Q19, MHD, 32-bit arithmetic)
+template <class S>
+static hipError_t init_ecr_impl(const LbmArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL((k_initialize<19, S, true, true>), dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template <class S>
+static hipError_t sc_ecr_impl(bool trt, bool lod, uint32_t block, const LbmArgs& a, hipStream_t s) {
+    const uint32_t g = blocks_for(a.N, block);
+    const size_t sh = lod ? (size_t)block * sizeof(float4) : 0;
+#define ION_SC(T, L) hipLaunchKernelGGL((k_stream_collide<19, S, true, T, L, true>), dim3(g), dim3(block), sh, s, a)
+    if (lod) { if (trt) ION_SC(true, true); else ION_SC(false, true); }
+    else { if (trt) ION_SC(true, false); else ION_SC(false, false); }
+#undef ION_SC
+    return hipGetLastError();
+}
+
 template <int Q, class S>
 static hipError_t sc_impl(bool mhd, bool trt, bool lod, uint32_t block, const LbmArgs& a, hipStream_t s) {
     const uint32_t g = blocks_for(a.N, block);
@@ -499,7 +566,14 @@ static hipError_t uf_impl(const LbmArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_initialize(int q, int store, bool mhd, const LbmArgs& a, hipStream_t s) {
+hipError_t launch_initialize(int q, int store, bool mhd, const LbmArgs& a, hipStream_t s, bool ecr) {
+    if (ecr) {
+        if (q != 19 || !mhd) return hipErrorInvalidValue;
+        if (store == STORE_FP32) return init_ecr_impl<F32>(a, s);
+        if (store == STORE_FP16S) return init_ecr_impl<F16S>(a, s);
+        if (store == STORE_FP16C) return init_ecr_impl<F16C>(a, s);
+        return hipErrorInvalidValue;
+    }
     if (store == STORE_FP64) return q == 19 ? init_impl<19, F64>(mhd, a, s) : hipErrorInvalidValue;
     if (q == 19) {
         if (store == STORE_FP32) return init_impl<19, F32>(mhd, a, s);
@@ -518,7 +592,14 @@ hipError_t launch_initialize(int q, int store, bool mhd, const LbmArgs& a, hipSt
 }
 
 hipError_t launch_stream_collide(int q, int store, bool mhd, bool trt, bool lod, uint32_t block, const LbmArgs& a,
-                                 hipStream_t s) {
+                                 hipStream_t s, bool ecr) {
+    if (ecr) {
+        if (q != 19 || !mhd) return hipErrorInvalidValue;
+        if (store == STORE_FP32) return sc_ecr_impl<F32>(trt, lod, block, a, s);
+        if (store == STORE_FP16S) return sc_ecr_impl<F16S>(trt, lod, block, a, s);
+        if (store == STORE_FP16C) return sc_ecr_impl<F16C>(trt, lod, block, a, s);
+        return hipErrorInvalidValue;
+    }
     if (store == STORE_FP64) return q == 19 ? sc_impl<19, F64>(mhd, trt, lod, block, a, s) : hipErrorInvalidValue;
     if (q == 19) {
         if (store == STORE_FP32) return sc_impl<19, F32>(mhd, trt, lod, block, a, s);

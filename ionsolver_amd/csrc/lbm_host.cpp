@@ -106,6 +106,10 @@ struct ion_handle {
     ion_surrogate_stats sstats{};
     // imported meshes (mesh_host.cpp)
     ion::MeshSet* meshes = nullptr;
+    // SUBGRID_ECR (domain.rs:200-211)
+    bool ecr = false;
+    void* eti = nullptr;
+    float *et = nullptr, *e_var = nullptr;
 
     size_t ddf_elem() const { return store == ion::STORE_FP64 ? 8 : (store == ion::STORE_FP32 ? 4 : 2); }
     bool f64() const { return store == ion::STORE_FP64; }
@@ -212,6 +216,11 @@ ion::LbmArgs lbm_args(ion_handle* h) {
     a.ffield = h->cfg.ext_force_field ? 1u : 0u;
     a.upd = h->cfg.graphics_active ? 1u : 0u;
     a.qin = h->cfg.ext_charge_inlet ? 1u : 0u;
+    a.eti = h->eti;
+    a.et = h->et;
+    a.e_var = h->e_var;
+    a.kkbme = h->kc.kkbme;
+    a.keabs = h->kc.keabs;
     a.lod_bx = h->lod_bx;
     a.fx = h->fx;
     a.fy = h->fy;
@@ -327,7 +336,7 @@ int do_step(ion_handle* h) {
     ion::LbmArgs a = lbm_args(h);
     rc = timed(h, 0, [&] {
         return ion::launch_stream_collide(h->Q, h->store, h->mhd, h->cfg.relaxation_time == ION_TRT, h->lod,
-                                          h->sc_block, a, h->stream);
+                                          h->sc_block, a, h->stream, h->ecr);
     });
     if (rc) return rc;
     if (h->mhd) {
@@ -361,7 +370,11 @@ int validate(const ion_config& c) {
     if (N > 0x40000000ull)
         return fail(ION_ERR_UNSUPPORTED, "more than 2^30 cells (1024^3) on one GPU: the kernels address every "
                                          "per-direction array with 32-bit byte offsets");
-    if (c.ext_subgrid_ecr) return fail(ION_ERR_UNSUPPORTED, "ext_subgrid_ecr is not supported");
+    if (c.ext_subgrid_ecr) {
+        if (!c.ext_magneto_hydro) return fail(ION_ERR_INVALID, "ext_subgrid_ecr needs ext_magneto_hydro");
+        if (c.velocity_set != ION_D3Q19 || c.float_type == ION_FP64)
+            return fail(ION_ERR_UNSUPPORTED, "ext_subgrid_ecr is built for D3Q19 with FP32, FP16S or FP16C storage");
+    }
     if (c.ext_force_field && !c.ext_volume_force) return fail(ION_ERR_INVALID, "ext_force_field needs ext_volume_force");
     if (c.ext_charge_inlet && !(c.ext_magneto_hydro && c.ext_equilibrium_boudaries))
         return fail(ION_ERR_INVALID, "ext_charge_inlet needs ext_magneto_hydro and ext_equilibrium_boudaries");
@@ -424,6 +437,7 @@ __attribute__((visibility("hidden"))) int ion_internal_view(ion_handle* h, ion::
     v->e_dyn = h->e_dyn;
     v->b_dyn = h->b_dyn;
     v->meshes = &h->meshes;
+    v->e_var = h->e_var;
     return ION_OK;
 }
 
@@ -466,6 +480,10 @@ int ion_kernel_constants_of(const ion_config* c, ion_kernel_constants* k) {
     const float kce = 1.0f;  // k_charge_expansion_lu, units.rs:169-173
     k->wq = 1.0f / (2.0f * kce + 0.5f);
     k->n_lod = c->ext_magneto_hydro ? (uint32_t)n_lod_own_of(c->mhd_lod_depth) : 0u;
+    // units.rs:187-193: f64 quotients cast to f32
+    const double m = c->units.m, s = c->units.s, kk = c->units.k, a = c->units.a, kg = c->units.kg;
+    k->kkbme = (float)(-22734499.72063751808909449412 / ((m * m) / ((s * s) * kk)));
+    k->keabs = (float)(1.40897016100511360652E-8 / ((a * a) * (s * s) / kg));
     return ION_OK;
 }
 
@@ -522,6 +540,12 @@ int ion_create(const ion_config* cfg, ion_handle** out) {
         if ((rc = dalloc(h, (void**)&h->src_vd, n_top * 4 * rs))) return bail(rc);
         if ((rc = dalloc(h, (void**)&h->src_count, sizeof(uint32_t)))) return bail(rc);
         if (h->lod && (rc = dalloc(h, (void**)&h->lod_seg, (N / h->lod_bx) * 4 * rs))) return bail(rc);
+        h->ecr = cfg->ext_subgrid_ecr != 0;
+        if (h->ecr) {
+            if ((rc = dalloc(h, &h->eti, 7 * N * de))) return bail(rc);
+            if ((rc = dalloc(h, (void**)&h->et, N * 4))) return bail(rc);
+            if ((rc = dalloc(h, (void**)&h->e_var, 3 * N * 4))) return bail(rc);
+        }
     }
     e = ion::launch_fill(h->rho, h->f64(), N, 1.0, h->stream);  // rho buffer starts at 1 (domain.rs:164)
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -546,7 +570,7 @@ int ion_initialize(ion_handle* h) {
     if (!h) return fail(ION_ERR_INVALID, "null handle");
     h->t = 1;  // initialize writes the odd-step layout (mod.rs:216)
     ion::LbmArgs a = lbm_args(h);
-    int rc = timed(h, 4, [&] { return ion::launch_initialize(h->Q, h->store, h->mhd, a, h->stream); });
+    int rc = timed(h, 4, [&] { return ion::launch_initialize(h->Q, h->store, h->mhd, a, h->stream, h->ecr); });
     if (rc) return rc;
     if (h->mhd && (rc = field_update(h))) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -611,6 +635,9 @@ static void* field_ptr(const ion_handle* h, int field, size_t* bytes, uint32_t* 
         case ION_FIELD_QU_LOD: p = h->qu_lod; b = p ? h->n_lod * 4 * rs : 0; e = (uint32_t)rs; break;
         case ION_FIELD_EI: p = h->ei; b = p ? N * h->Q * de : 0; e = (uint32_t)de; break;
         case ION_FIELD_FIELD_SUMS: p = h->field_raw; b = p ? 6 * N * 4 : 0; break;
+        case ION_FIELD_E_VAR: p = h->e_var; b = p ? 3 * N * 4 : 0; break;
+        case ION_FIELD_ETI: p = h->eti; b = p ? 7 * N * de : 0; e = (uint32_t)de; break;
+        case ION_FIELD_ET: p = h->et; b = p ? N * 4 : 0; break;
         default: b = 0;
     }
     if (bytes) *bytes = b;

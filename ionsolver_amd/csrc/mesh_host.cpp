@@ -367,6 +367,20 @@ int ion_psi_math_check(uint64_t* sqrt_mismatch, uint64_t* div_mismatch, uint64_t
     return ION_OK;
 }
 
+int ion_precompute_e_ecr(ion_handle* h) {
+    HandleView v;
+    int rc = ion_internal_view(h, &v);
+    if (rc) return rc;
+    if (!v.e_var) return fail(ION_ERR_INVALID, "precompute_E_ECR needs ext_magneto_hydro and ext_subgrid_ecr");
+    SourceList L;
+    if ((rc = select_sources(v, 1, &L))) return rc;
+    mesh::Sources s{L.pos, L.val, L.total};
+    hipError_t e = mesh::launch_static_e(s, v.flags, v.e_var, v.nx, v.ny, v.nz, v.kc->ke, v.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
+    if (e != hipSuccess) return fail(ION_ERR_DEVICE, std::string("precompute_E_ECR: ") + hipGetErrorString(e));
+    return ION_OK;
+}
+
 int ion_precompute_e(ion_handle* h) {
     HandleView v;
     int rc = ion_internal_view(h, &v);

@@ -73,6 +73,9 @@ class Field(enum.IntEnum):
     QU_LOD = 11
     EI = 12
     FIELD_SUMS = 13   # raw direct source sums (E then B, 6 x N), once a surrogate exists
+    E_VAR = 14        # ext_subgrid_ecr: oscillating electric field (3 x N)
+    ETI = 15          # ext_subgrid_ecr: electron temperature DDFs (7 x N storage codes)
+    ET = 16           # ext_subgrid_ecr: electron temperature (N)
 
 
 class FieldMode(enum.IntEnum):
@@ -107,7 +110,7 @@ class _Config(ctypes.Structure):
 class _KernelConstants(ctypes.Structure):
     _fields_ = [("w", ctypes.c_float), ("c", ctypes.c_float), ("ke", ctypes.c_float), ("kmu", ctypes.c_float),
                 ("kmu0", ctypes.c_float), ("kkge", ctypes.c_float), ("wq", ctypes.c_float),
-                ("n_lod", ctypes.c_uint32)]
+                ("n_lod", ctypes.c_uint32), ("kkbme", ctypes.c_float), ("keabs", ctypes.c_float)]
 
 
 class _Timing(ctypes.Structure):
@@ -184,6 +187,7 @@ ABI = {
     "ion_voxelise_mesh": (_I, [_VP, ctypes.c_uint32, _I, _F, _F, _F]),
     "ion_precompute_b": (_I, [_VP]),
     "ion_precompute_e": (_I, [_VP]),
+    "ion_precompute_e_ecr": (_I, [_VP]),
     "ion_psi_math_check": (_I, [ctypes.POINTER(_U64)] * 3),
     "ion_surrogate_get_stats": (_I, [_VP, ctypes.POINTER(_SurStats)]),
     "ion_surrogate_param_count": (_I, [_VP, ctypes.POINTER(_U64)]),
@@ -532,6 +536,10 @@ class Lbm:
     def precompute_e(self):
         """Lbm::precompute_E (src/lbm/mod.rs:302)."""
         _check(self._lib.ion_precompute_e(self._h))
+
+    def precompute_e_ecr(self):
+        """Lbm::precompute_E_ECR (src/lbm/mod.rs:319): the static E of charged meshes into E_var."""
+        _check(self._lib.ion_precompute_e_ecr(self._h))
 
 
 def hbm_copy_probe(nbytes=1 << 30, reps=10):

@@ -98,11 +98,24 @@ def mhd_blobs(n=256, seed=1234, float_type=FloatType.FP32, lod_depth=4, n_blobs=
     return cfg, arrays, blobs
 
 
+def add_subgrid_ecr(cfg, arrays, et0=1e-3, e_amp=1e-3):
+    """Switch on SUBGRID_ECR for a magnetised setup: a uniform initial
+    electron temperature et0 and an oscillating field E_var of magnitude
+    e_amp along x (lattice units), the direction a microwave feed would give."""
+    n = cfg.n_x * cfg.n_y * cfg.n_z
+    cfg.ext_subgrid_ecr = True
+    e_var = np.zeros(3 * n, np.float32)
+    e_var[:n] = e_amp
+    arrays["e_var"] = e_var
+    arrays["et"] = np.full(n, et0, np.float32)
+    return cfg, arrays
+
+
 def apply(lbm, arrays):
     """Write setup arrays into the domain buffers (the setups' bwrite! calls)."""
     from .lbm import Field
     names = dict(flags=Field.FLAGS, rho=Field.RHO, u=Field.U, q=Field.Q, e_stat=Field.E_STAT, b_stat=Field.B_STAT,
-                 F=Field.F)
+                 F=Field.F, e_var=Field.E_VAR, et=Field.ET)
     for k, v in arrays.items():
         lbm.write_field(names[k], v)
 

@@ -300,6 +300,7 @@ struct SimBase {
     virtual double sum_q() = 0;
     virtual uint64_t n_lod() const = 0;
     bool mhd() const { return (p.ext & ORC_EXT_MAGNETO_HYDRO) != 0; }
+    bool ecr() const { return mhd() && (p.ext & ORC_EXT_SUBGRID_ECR) != 0; }
 
     /* Lbm::initialize, src/lbm/mod.rs:214-231 (single domain: the
      * communicate_* calls are no-ops). */
@@ -333,11 +334,11 @@ struct Sim : SimBase {
     uint32_t D = 0, nd = 1;
     uint64_t n_lod_own = 1;
     uint32_t bx = 1, by = 1, bz = 1; /* LOD block size */
-    std::vector<T> fi, ei, fqi;
-    std::vector<R> rho, u, F, e_stat, b_stat, e_dyn, b_dyn, q, qu_lod;
+    std::vector<T> fi, ei, fqi, eti;
+    std::vector<R> rho, u, F, e_stat, b_stat, e_dyn, b_dyn, q, qu_lod, e_var, et;
     std::vector<uint8_t> flags;
     std::vector<R> lod_cell; /* per-cell LOD contributions (4 per cell) */
-    R W, C, KE, KMU, KKGE, WQ, FX, FY, FZ;
+    R W, C, KE, KMU, KKGE, WQ, FX, FY, FZ, KKBME, KEABS;
 
     bool ext(uint32_t bit) const { return (p.ext & bit) != 0; }
 
@@ -348,6 +349,7 @@ struct Sim : SimBase {
         W = R(p.w); C = R(0.57735027f);
         KE = R(p.ke); KMU = R(p.kmu); KKGE = R(p.kkge); WQ = R(p.wq);
         FX = R(p.fx); FY = R(p.fy); FZ = R(p.fz);
+        KKBME = R(p.kkbme); KEABS = R(p.keabs);
         fi.assign(N * Q, T(0));
         rho.assign(N, R(1));
         u.assign(3 * N, R(0));
@@ -365,6 +367,11 @@ struct Sim : SimBase {
             ei.assign(N * Q, T(0));
             q.assign(N, R(0));
             qu_lod.assign(n_lod_own * 4, R(0));
+            if (ecr()) {
+                eti.assign(N * 7, T(0));
+                et.assign(N, R(0));
+                e_var.assign(3 * N, R(0));
+            }
             if (D > 0) {
                 bx = NX / nd; by = NY / nd; bz = NZ / nd;
                 lod_cell.assign(N * 4, R(0));
@@ -394,6 +401,9 @@ struct Sim : SimBase {
             case ORC_BUF_Q: return put(q.data(), q.size(), sizeof(R));
             case ORC_BUF_QU_LOD: return put(qu_lod.data(), qu_lod.size(), sizeof(R));
             case ORC_BUF_EI: return put(ei.data(), ei.size(), sizeof(T));
+            case ORC_BUF_E_VAR: return put(e_var.data(), e_var.size(), sizeof(R));
+            case ORC_BUF_ETI: return put(eti.data(), eti.size(), sizeof(T));
+            case ORC_BUF_ET: return put(et.data(), et.size(), sizeof(R));
         }
         return put(nullptr, 0, 0);
     }
@@ -484,8 +494,66 @@ struct Sim : SimBase {
                 }
                 f_eq<Q, R>(R(0), u[n], u[N + n], u[2 * N + n], feq);
                 store<Q>(n, feq, ei.data(), j, tt);
+                if (ecr()) { /* electron temperature DDFs (:827-831) */
+                    a_eq<R>(et[n], u[n], u[N + n], u[2 * N + n], qeq);
+                    store<7>(n, qeq, eti.data(), j7, tt);
+                }
             }
         }
+    }
+
+    /* length(v) as sqrt(dot(v, v)), dot as (x x + y y) + z z */
+    static R len3(R x, R y, R z) { return std::sqrt((x * x + y * y) + z * z); }
+
+    /* grad_mag_v(n, B_dyn), sim_kernels.cl:453-461: zero on the lattice
+     * faces, else per axis mag(c + e) - mag(c - e) / 2 (as written: only the
+     * second term is halved) */
+    void grad_mag_b(uint32_t n, R* g) const {
+        const uint32_t x = n % NX, y = (n / NX) % NY, z = (uint32_t)(n / ((uint64_t)NX * NY));
+        if (x == 0 || x == NX - 1 || y == 0 || y == NY - 1 || z == 0 || z == NZ - 1) {
+            g[0] = g[1] = g[2] = R(0);
+            return;
+        }
+        auto mag = [&](uint64_t i) { return len3(b_dyn[i], b_dyn[N + i], b_dyn[2 * N + i]); };
+        const uint64_t sy = NX, sz = (uint64_t)NX * NY;
+        g[0] = mag(n + 1) - mag(n - 1) / R(2);
+        g[1] = mag(n + sy) - mag(n - sy) / R(2);
+        g[2] = mag(n + sz) - mag(n - sz) / R(2);
+    }
+
+    /* SUBGRID_ECR for one cell, sim_kernels.cl:557-629: electron temperature
+     * streaming, ECR heating by the components of E_var normal to B, the
+     * guiding-centre drift along -grad|B|, temperature collision, ionisation.
+     * The cyclotron frequency and relative absorption of :582-585 are
+     * computed there but never used, so they are not restated. */
+    void ecr_cell(uint32_t n, const uint32_t* j7, uint64_t tt, R Bx, R By, R Bz, R& rhon_e, R& uxn_e, R& uyn_e,
+                  R& uzn_e, R& rhon_q) {
+        R ethn[7];
+        load<7>(n, ethn, eti.data(), j7, tt);
+        R Etn = R(0);
+        for (int i = 0; i < 7; i++) Etn += ethn[i];
+        Etn += R(1);
+        const R Evx = e_var[n], Evy = e_var[N + n], Evz = e_var[2 * N + n];
+        const R lb = len3(Bx, By, Bz);
+        const R s = ((Evx * Bx + Evy * By) + Evz * Bz) / (lb * lb);
+        const R env = len3(Evx - s * Bx, Evy - s * By, Evz - s * Bz);
+        Etn += KEABS / KKBME * (rhon_e + R(0.00001f)) / KKGE * (env * env);
+        R g[3];
+        grad_mag_b(n, g);
+        const R kt = KKBME * Etn;
+        const R dx = kt * g[0] / lb, dy = kt * g[1] / lb, dz = kt * g[2] / lb;
+        uxn_e += dx;
+        uyn_e += dy;
+        uzn_e += dz;
+        Etn -= len3(dx, dy, dz) / KKBME;
+        R eteq[7];
+        a_eq<R>(Etn, uxn_e, uyn_e, uzn_e, eteq);
+        if (ext(ORC_EXT_UPDATE_FIELDS)) et[n] = Etn;
+        for (int i = 0; i < 7; i++) ethn[i] = std::fma(R(1) - WQ, ethn[i], WQ * eteq[i]);
+        store<7>(n, ethn, eti.data(), j7, tt);
+        const R dq = R(0.0001f) * Etn;
+        rhon_e += dq;
+        rhon_q += dq;
     }
 
     /* a5: stream_collide for one cell, sim_kernels.cl:465-758. */
@@ -532,6 +600,7 @@ struct Sim : SimBase {
             R rhon_q = R(0);
             for (int i = 0; i < 7; i++) rhon_q += qhn[i];
             rhon_q += R(1);
+            if (ecr()) ecr_cell(n, j7, tt, Bx, By, Bz, rhon_e, uxn_e, uyn_e, uzn_e, rhon_q);
             /* charge inlet (ORC_EXT_CHARGE_INLET): the cell's charge is held */
             const bool inlet = is_e && ext(ORC_EXT_CHARGE_INLET);
             if (inlet) rhon_q = q[n];

@@ -48,6 +48,9 @@ enum {
      * charge density from q[n], reset their D3Q7 DDFs to a_eq(q[n], u) with
      * no collision and leave q[n] unwritten. */
     ORC_EXT_CHARGE_INLET = 1 << 5,
+    /* SUBGRID_ECR (sim_kernels.cl:557-629, 827-831): electron temperature
+     * DDFs, ECR heating by E_var, gyration drift, ionisation. */
+    ORC_EXT_SUBGRID_ECR = 1 << 6,
 };
 
 /* Storage formats: discriminants of FloatType (src/lbm/types.rs:75-83), plus
@@ -70,6 +73,7 @@ typedef struct orc_params {
     float w;                  /* DEF_W = 1/(3 nu + 1/2) */
     float fx, fy, fz;         /* volume force kernel arguments */
     float ke, kmu, kkge, wq;  /* DEF_KE, DEF_KMU, DEF_KKGE, DEF_WQ */
+    float kkbme, keabs;       /* DEF_KKBME, DEF_KEABS (SUBGRID_ECR) */
 } orc_params;
 
 /* Buffer ids, one per reference device buffer (src/lbm/domain.rs:151-211). */
@@ -77,6 +81,7 @@ enum {
     ORC_BUF_FI = 0, ORC_BUF_RHO, ORC_BUF_U, ORC_BUF_FLAGS, ORC_BUF_F,
     ORC_BUF_E_STAT, ORC_BUF_B_STAT, ORC_BUF_E_DYN, ORC_BUF_B_DYN,
     ORC_BUF_FQI, ORC_BUF_Q, ORC_BUF_QU_LOD, ORC_BUF_EI,
+    ORC_BUF_E_VAR, ORC_BUF_ETI, ORC_BUF_ET, /* SUBGRID_ECR */
     ORC_BUF_COUNT
 };
 

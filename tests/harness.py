@@ -23,7 +23,7 @@ from ionsolver_amd import Field  # noqa: E402
 
 FIELD_OF = dict(fi=Field.FI, rho=Field.RHO, u=Field.U, flags=Field.FLAGS, F=Field.F, e_stat=Field.E_STAT,
                 b_stat=Field.B_STAT, e_dyn=Field.E_DYN, b_dyn=Field.B_DYN, fqi=Field.FQI, q=Field.Q,
-                qu_lod=Field.QU_LOD, ei=Field.EI)
+                qu_lod=Field.QU_LOD, ei=Field.EI, e_var=Field.E_VAR, eti=Field.ETI, et=Field.ET)
 
 
 def oracle_for(cfg, float_type=None):
@@ -42,11 +42,13 @@ def oracle_for(cfg, float_type=None):
         ext |= O.EXT_UPDATE_FIELDS
     if cfg.ext_charge_inlet:
         ext |= O.EXT_CHARGE_INLET
+    if cfg.ext_subgrid_ecr:
+        ext |= O.EXT_SUBGRID_ECR
     ft = int(cfg.float_type) if float_type is None else float_type
     return O.OracleSim(cfg.n_x, cfg.n_y, cfg.n_z, velocity_set=int(cfg.velocity_set),
                        relaxation_time=int(cfg.relaxation_time), float_type=ft, lod_depth=cfg.mhd_lod_depth,
                        ext=ext, w=k["w"], fx=cfg.f_x, fy=cfg.f_y, fz=cfg.f_z, ke=k["ke"], kmu=k["kmu"],
-                       kkge=k["kkge"], wq=k["wq"])
+                       kkge=k["kkge"], wq=k["wq"], kkbme=k["kkbme"], keabs=k["keabs"])
 
 
 def random_inputs(cfg, rng, solid_frac=0.1, e_frac=0.05, charge=0.05, field=0.05):
@@ -71,6 +73,9 @@ def random_inputs(cfg, rng, solid_frac=0.1, e_frac=0.05, charge=0.05, field=0.05
         arrays["q"] = q.astype(np.float32)
         arrays["e_stat"] = (field * rng.standard_normal(3 * n)).astype(np.float32)
         arrays["b_stat"] = (field * rng.standard_normal(3 * n)).astype(np.float32)
+    if cfg.ext_subgrid_ecr:
+        arrays["e_var"] = (field * rng.standard_normal(3 * n)).astype(np.float32)
+        arrays["et"] = (0.05 * rng.uniform(size=n)).astype(np.float32)
     return arrays
 
 
@@ -83,7 +88,7 @@ def write_both(gpu, orc, arrays):
 
 def make_config(nx=16, ny=16, nz=16, velocity_set=2, float_type=2, relaxation_time=0, mhd=False, lod_depth=2,
                 eqb=True, vforce=True, ffield=False, graphics=True, force=(1e-5, -2e-5, 3e-5), nu=0.02, units=None,
-                charge_inlet=False):
+                charge_inlet=False, ecr=False):
     cfg = I.LbmConfig()
     cfg.velocity_set = I.VelocitySet(velocity_set)
     cfg.float_type = I.FloatType(float_type)
@@ -96,6 +101,7 @@ def make_config(nx=16, ny=16, nz=16, velocity_set=2, float_type=2, relaxation_ti
     cfg.ext_magneto_hydro = mhd
     cfg.mhd_lod_depth = lod_depth
     cfg.ext_charge_inlet = charge_inlet
+    cfg.ext_subgrid_ecr = ecr
     cfg.graphics_config.graphics_active = graphics
     if vforce or mhd:
         cfg.f_x, cfg.f_y, cfg.f_z = force
@@ -120,6 +126,8 @@ def fields_of(cfg):
         names.append("F")
     if cfg.ext_magneto_hydro:
         names += ["e_stat", "b_stat", "e_dyn", "b_dyn", "fqi", "q", "qu_lod", "ei"]
+        if cfg.ext_subgrid_ecr:
+            names += ["e_var", "eti", "et"]
     return names
 
 

@@ -18,13 +18,14 @@ EXT_FORCE_FIELD = 1 << 2
 EXT_MAGNETO_HYDRO = 1 << 3
 EXT_UPDATE_FIELDS = 1 << 4
 EXT_CHARGE_INLET = 1 << 5
+EXT_SUBGRID_ECR = 1 << 6
 
 FP16S, FP16C, FP32, FP64 = 0, 1, 2, 3
 D2Q9, D3Q15, D3Q19, D3Q27 = 0, 1, 2, 3
 Q_OF = {D2Q9: 9, D3Q15: 15, D3Q19: 19}
 
 BUF = dict(fi=0, rho=1, u=2, flags=3, F=4, e_stat=5, b_stat=6, e_dyn=7, b_dyn=8,
-           fqi=9, q=10, qu_lod=11, ei=12)
+           fqi=9, q=10, qu_lod=11, ei=12, e_var=13, eti=14, et=15)
 
 
 class OrcParams(ctypes.Structure):
@@ -35,6 +36,7 @@ class OrcParams(ctypes.Structure):
         ("lod_depth", ctypes.c_uint32), ("ext", ctypes.c_uint32),
         ("w", ctypes.c_float), ("fx", ctypes.c_float), ("fy", ctypes.c_float), ("fz", ctypes.c_float),
         ("ke", ctypes.c_float), ("kmu", ctypes.c_float), ("kkge", ctypes.c_float), ("wq", ctypes.c_float),
+        ("kkbme", ctypes.c_float), ("keabs", ctypes.c_float),
     ]
 
 
@@ -163,9 +165,9 @@ class OracleSim:
     """One oracle simulation (mirrors the reference Lbm of one domain)."""
 
     def __init__(self, nx, ny, nz, velocity_set=D3Q19, relaxation_time=0, float_type=FP32, lod_depth=4,
-                 ext=0, w=1.0, fx=0.0, fy=0.0, fz=0.0, ke=0.0, kmu=0.0, kkge=0.0, wq=0.4):
+                 ext=0, w=1.0, fx=0.0, fy=0.0, fz=0.0, ke=0.0, kmu=0.0, kkge=0.0, wq=0.4, kkbme=0.0, keabs=0.0):
         p = OrcParams(velocity_set, relaxation_time, float_type, nx, ny, nz, lod_depth, ext,
-                      w, fx, fy, fz, ke, kmu, kkge, wq)
+                      w, fx, fy, fz, ke, kmu, kkge, wq, kkbme, keabs)
         err = ctypes.create_string_buffer(256)
         h = lib().orc_create(ctypes.byref(p), err)
         if not h:

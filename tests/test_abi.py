@@ -99,7 +99,9 @@ def base_config(**kw):
     (dict(n_x=0), -1),
     (dict(ext_magneto_hydro=True), -1),                      # needs ext_volume_force
     (dict(ext_magneto_hydro=True, ext_volume_force=True, n_x=20), -2),  # 20 not divisible by 2^4
-    (dict(ext_subgrid_ecr=True), -2),
+    (dict(ext_subgrid_ecr=True), -1),                        # needs ext_magneto_hydro
+    (dict(ext_subgrid_ecr=True, ext_magneto_hydro=True, ext_volume_force=True,
+          velocity_set=I.VelocitySet.D3Q15), -2),             # built for D3Q19
     (dict(ext_force_field=True), -1),
 ])
 def test_invalid_configs_are_rejected_before_touching_the_device(kw, code):

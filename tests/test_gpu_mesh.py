@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 STL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "stl")
 
 
-def make_lbm(shape, mhd=True, lbm_length=None):
+def make_lbm(shape, mhd=True, lbm_length=None, ecr=False):
     cfg = I.LbmConfig()
     cfg.velocity_set = I.VelocitySet.D3Q19
     cfg.float_type = I.FloatType.FP32
@@ -32,6 +32,7 @@ def make_lbm(shape, mhd=True, lbm_length=None):
     cfg.ext_volume_force = mhd
     cfg.ext_magneto_hydro = mhd
     cfg.mhd_lod_depth = 1
+    cfg.ext_subgrid_ecr = ecr
     if lbm_length is not None:  # setup_deeva_test units (src/setup.rs:397) at lattice length lbm_length
         cfg.units.set(lbm_length, 1.0, 1.0, 1.0, 1.0, 0.1, 1.0, 10e-8, 1.0, 50000.0)
     return I.Lbm(cfg)
@@ -128,6 +129,37 @@ def test_voxelize_and_precompute_synthetic(tmp_path):
     # lattice lengths in metres of one cell = 1 (units default), so the meshes are placed as written
     lbm, flags = run_both(shape, meshes)
     assert np.count_nonzero(flags & 0x10) > 400 and np.count_nonzero((flags & 0x1F) == 0x05) > 10
+
+
+def test_precompute_e_ecr_writes_e_var(tmp_path):
+    """precompute_E_ECR (src/lbm/mod.rs:319-331, domain.rs:569-578) runs the
+    static_e_from_mesh kernel into E_var instead of E_stat: E_var (zero
+    after allocation) gains the Coulomb field of the charged cells, E_stat
+    is untouched; without ext_subgrid_ecr the call is rejected."""
+    shape = (20, 18, 16)
+    lbm = make_lbm(shape, ecr=True)
+    n = shape[0] * shape[1] * shape[2]
+    rng = np.random.default_rng(9)
+    e_stat0 = (1e-3 * rng.standard_normal(3 * n)).astype(np.float32)
+    lbm.write_field(Field.E_STAT, e_stat0)
+    flags = np.zeros(n, np.uint8)
+    b_dyn = np.zeros(3 * n, np.float32)
+    k = lbm.config.kernel_constants()
+    units = lbm.config.units
+    p = write_stl(tmp_path / "p.stl", box((4.45, 3.7, 5.2), (14.85, 7.3, 9.6)))
+    i = lbm.import_mesh(p, 1.0, 0.0, 0.0, 0.0)
+    v, b = lbm.mesh(i)
+    lbm.voxelise_mesh(i, ModelType.ChargedECR, (3.0e-13, 0.0, 0.0))
+    c = np.float32(3.0e-13) / (np.float32(units.a) * np.float32(units.s))
+    O.voxelize(shape, flags, b_dyn, v, b["p_min"], b["p_max"], O.MODEL_FLAG[int(ModelType.ChargedECR)], (c, 0.0, 0.0))
+    assert np.array_equal(lbm.read_field(Field.FLAGS), flags) and np.count_nonzero(flags) > 50
+    lbm.precompute_e_ecr()
+    e_var = np.zeros(3 * n, np.float32)
+    O.precompute_e(shape, flags, b_dyn, e_var, k["ke"])
+    assert np.array_equal(lbm.read_field(Field.E_VAR), e_var) and np.abs(e_var).max() > 0
+    assert np.array_equal(lbm.read_field(Field.E_STAT), e_stat0)
+    with pytest.raises(I.IonError, match="subgrid_ecr"):
+        make_lbm(shape).precompute_e_ecr()
 
 
 def test_deeva_thruster_at_reduced_resolution():

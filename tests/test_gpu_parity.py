@@ -63,6 +63,17 @@ def test_exact_field_on_lod_tiles(shape, D):
     assert compare_all(gpu, orc, info) == []
 
 
+@pytest.mark.parametrize("float_type,trt,D", [(2, 0, 2), (1, 0, 2), (0, 1, 1), (2, 0, 0)])
+def test_mhd_subgrid_ecr(float_type, trt, D):
+    """SUBGRID_ECR (electron temperature, ECR heating by E_var, gyration
+    drift, ionisation), every buffer exact."""
+    rng = np.random.default_rng(700 + 10 * float_type + trt + D)
+    gpu, orc, info = make_pair(rng, nx=16, ny=16, nz=16, mhd=True, lod_depth=D, float_type=float_type,
+                               relaxation_time=trt, ecr=True)
+    run_both(gpu, orc, 3)
+    assert compare_all(gpu, orc, info) == []
+
+
 def test_mhd_lod_depth0_and_d3q15():
     rng = np.random.default_rng(400)
     gpu, orc, info = make_pair(rng, nx=8, ny=8, nz=8, mhd=True, lod_depth=0, float_type=2)
