@@ -408,6 +408,18 @@ def main():
                                     "a second buffer, nontemporal, best of 10 (read + write)"}
     kernels["stream_collide"]["frac_of_pattern_probe"] = round(kernels["stream_collide"]["GB/s"] / pat_gbs, 4) \
         if kernels["stream_collide"]["GB/s"] else None
+    # the same lattice with stream_collide's in-place esoteric-pull addressing
+    # (half the arrays at the neighbour n + c_i) and no arithmetic: the access
+    # pattern's own ceiling
+    if N <= (1 << 27) and cfg.velocity_set.name == "D3Q19":
+        _, pull_gbs = I.hbm_pull_probe(elem, streams, cfg.n_x, cfg.n_y, cfg.n_z, 10)
+        out["pull_probe"] = {"GB/s": round(pull_gbs, 1), "frac_of_peak": round(pull_gbs / HBM_PEAK_GBS, 4),
+                             "streams": streams, "bytes_per_element": elem, "lattice": [cfg.n_x, cfg.n_y, cfg.n_z],
+                             "note": "one work-item per cell reading every SoA array at the cell or at n + c_i "
+                                     "(esoteric pull, even step), then rewriting the same slots, nontemporal, "
+                                     "best of 10 (read + write)"}
+        kernels["stream_collide"]["frac_of_pull_probe"] = round(kernels["stream_collide"]["GB/s"] / pull_gbs, 4) \
+            if kernels["stream_collide"]["GB/s"] else None
     if thruster:
         out["surrogate"] = {k2: v for k2, v in lbm.surrogate_stats().items()}
         out["surrogate"]["note"] = ("field steps of the whole run including warm-up; the first is the direct "

@@ -275,6 +275,15 @@ int ion_hbm_copy_probe(uint64_t bytes, uint32_t reps, double* best_ms, double* g
  * reps + 1 runs, rate in GB/s (read + write). */
 int ion_hbm_stream_probe(uint32_t elem_bytes, uint32_t streams, uint32_t cells, uint32_t reps, double* best_ms,
                          double* gb_per_s);
+/* Measurement helper (no reference counterpart): stream_collide's in-place
+ * esoteric-pull addressing without its arithmetic, on an nx x ny x nz lattice.
+ * `streams` (19: fi; 45: fi, ei, fqi) SoA arrays of `elem_bytes` (2 or 4);
+ * one work-item per cell reads every array at the cell or, for the odd
+ * directions i, at the periodic neighbour n + c_i (the layout of an even
+ * step), then rewrites the same slots; best of reps + 1 runs, rate in GB/s
+ * (read + write). */
+int ion_hbm_pull_probe(uint32_t elem_bytes, uint32_t streams, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t reps,
+                       double* best_ms, double* gb_per_s);
 
 /* ------------------------------------------------------------------------
  * Meshes and static fields (src/mesh.rs, src/lbm/mod.rs:284-314).

@@ -181,6 +181,7 @@ ABI = {
     "ion_surrogate_set_learning_rate": (_I, [_VP, ctypes.c_float, ctypes.c_float]),
     "ion_hbm_copy_probe": (_I, [_U64, ctypes.c_uint32, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "ion_hbm_stream_probe": (_I, [ctypes.c_uint32] * 4 + [ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "ion_hbm_pull_probe": (_I, [ctypes.c_uint32] * 6 + [ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "ion_import_mesh": (_I, [_VP, ctypes.c_char_p] + [_F] * 7 + [ctypes.POINTER(ctypes.c_uint32)]),
     "ion_import_mesh_reposition": (_I, [_VP, ctypes.c_char_p] + [_F] * 7 + [ctypes.POINTER(ctypes.c_uint32)]),
     "ion_mesh_get": (_I, [_VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), _VP, _VP]),
@@ -554,6 +555,15 @@ def hbm_stream_probe(elem_bytes, streams, cells, reps=10):
     ms, gbs = ctypes.c_double(), ctypes.c_double()
     _check(load_library().ion_hbm_stream_probe(int(elem_bytes), int(streams), int(cells), int(reps),
                                                ctypes.byref(ms), ctypes.byref(gbs)))
+    return ms.value, gbs.value
+
+
+def hbm_pull_probe(elem_bytes, streams, nx, ny, nz, reps=10):
+    """Best (ms, GB/s read + write) of stream_collide's in-place esoteric-pull
+    addressing without arithmetic on an nx x ny x nz lattice."""
+    ms, gbs = ctypes.c_double(), ctypes.c_double()
+    _check(load_library().ion_hbm_pull_probe(int(elem_bytes), int(streams), int(nx), int(ny), int(nz), int(reps),
+                                             ctypes.byref(ms), ctypes.byref(gbs)))
     return ms.value, gbs.value
 
 
