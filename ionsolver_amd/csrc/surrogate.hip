@@ -141,15 +141,45 @@ __global__ void __launch_bounds__(256) k_gemm_nt(const __bf16* __restrict__ A, c
 // 128 x 128 tile per 256-thread block, 4 waves of 64 x 64 (4 x 4 MFMA tiles);
 // the next K slice is fetched into registers while the current one is
 // multiplied out of LDS.
+//
+// Tile order (m_inner): when B has more tiles than A — the head, whose B
+// (weights, N = 6 x cells per LOD block rows) is 2 GB at 1024^3 against one
+// A row per LOD block — the blocks of one XCD walk all M tiles of one N tile
+// before the next. Workgroups go to the 8 XCDs round robin (block b runs on
+// XCD b % 8), so each B tile is fetched from HBM once per XCD and then served
+// by that XCD's L2, and A is re-read from the caches. Otherwise N tiles are
+// innermost.
 constexpr int BM2 = 128, BN2 = 128;
+constexpr uint32_t XCDS = 8;
 
 template <int EPI>
-__global__ void __launch_bounds__(256) k_gemm_nt_128(const __bf16* __restrict__ A, const __bf16* __restrict__ B, int M,
-                                                     int N, int K, int lda, int ldb, GemmOut out) {
-    __shared__ __bf16 As[BM2][BK + LDS_PAD];
-    __shared__ __bf16 Bs[BN2][BK + LDS_PAD];
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_gemm_nt_128(const __bf16* __restrict__ A, const __bf16* __restrict__ B, int M,
+                                                     int N, int K, int lda, int ldb, GemmOut out, int m_inner) {
+    // The K-loop tiles (As, Bs) and the epilogue's staging (Cs, Fs) share one
+    // LDS allocation: the epilogue's first barrier follows the last MFMA
+    // reads. 24.3 KB instead of 45 KB per block, so twice as many blocks fit
+    // on a CU.
+    typedef __bf16 Tile[BM2][BK + LDS_PAD];
+    typedef float CTile[4][16][64 + 4];
+    typedef float FTile[4][16][FAR_PTS];
+    constexpr size_t SMEM = 2 * sizeof(Tile) > sizeof(CTile) + sizeof(FTile) ? 2 * sizeof(Tile)
+                                                                           : sizeof(CTile) + sizeof(FTile);
+    __shared__ __align__(16) unsigned char smem[SMEM];
+    Tile& As = *reinterpret_cast<Tile*>(smem);
+    Tile& Bs = *reinterpret_cast<Tile*>(smem + sizeof(Tile));
+    const uint32_t MT = (uint32_t)(M + BM2 - 1) / BM2, NT = (uint32_t)(N + BN2 - 1) / BN2;
+    uint32_t tile_m, tile_n;
+    if (m_inner) {
+        const uint32_t q = blockIdx.x / XCDS;
+        tile_m = q % MT;
+        tile_n = (q / MT) * XCDS + blockIdx.x % XCDS;
+    } else {
+        tile_n = blockIdx.x % NT;
+        tile_m = blockIdx.x / NT;
+    }
+    if (tile_n >= NT) return;  // padding of the last group of N tiles (the whole block)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.y * BM2, n0 = blockIdx.x * BN2;
+    const int m0 = (int)tile_m * BM2, n0 = (int)tile_n * BN2;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     f32x4 acc[4][4];
 #pragma unroll
@@ -194,12 +224,12 @@ __global__ void __launch_bounds__(256) k_gemm_nt_128(const __bf16* __restrict__ 
     // time: the slice is written in the MFMA layout and read back row-major,
     // so that consecutive lanes handle consecutive columns (coalesced stores)
     // and the per-output epilogue runs in a loop instead of 64 unrolled copies.
-    __shared__ float Cs[4][16][64 + 4];
+    CTile& Cs = *reinterpret_cast<CTile*>(smem);
     // Head with 64-column-aligned components (cells per block a multiple of
     // 64): a wave's 64 columns share the component, so the 27 far-field
     // samples of each of its rows are staged in LDS (one broadcast read per
     // sample) and each lane's 27 interpolation weights stay in registers.
-    __shared__ float Fs[4][16][FAR_PTS];
+    FTile& Fs = *reinterpret_cast<FTile*>(smem + sizeof(CTile));
     constexpr bool head_lds = EPI == EPI_HEAD;  // launched only with cells per block % 64 == 0
     float Lx[3], Ly[3], Lz[3];
     uint32_t hc = 0, hcell0 = 0;
@@ -235,13 +265,18 @@ __global__ void __launch_bounds__(256) k_gemm_nt_128(const __bf16* __restrict__ 
             const bool train = out.fine != nullptr;
             const float sc = hc < 3 ? out.scale_e : out.scale_b;
             const float kf = hc < 3 ? out.ke : out.kmu;
-            const float* stat = hc < 3 ? out.e_stat : out.b_stat;
-            float* dyn = hc < 3 ? out.e_dyn : out.b_dyn;
+            const float* stat = (hc < 3 ? out.e_stat : out.b_stat) + (size_t)(hc % 3) * out.ncell;
+            float* dyn = (hc < 3 ? out.e_dyn : out.b_dyn) + (size_t)(hc % 3) * out.ncell;
+            float* fine = train ? out.fine + (size_t)hc * out.ncell : nullptr;
+            // LOD block m -> its first cell: nd is a power of two and cell
+            // indices fit 32 bits (at most 2^30 cells)
+            const uint32_t lnd = (uint32_t)__builtin_ctz(out.nd), mnd = out.nd - 1u;
+            const uint32_t sxb = out.bx, syb = out.by * out.nx, szb = out.bz * out.nx * out.ny;
             // rows in batches of 8: all loads of a batch (flags, static
             // field) are issued before its first store
 #pragma unroll
             for (int tb = 0; tb < 16; tb += 8) {
-                uint64_t cell[8];
+                uint32_t cell[8];
                 bool valid[8];
                 uint8_t fl[8];
                 float st[8];
@@ -250,15 +285,13 @@ __global__ void __launch_bounds__(256) k_gemm_nt_128(const __bf16* __restrict__ 
                     const int m_raw = m0 + wm + i * 16 + tb + u;
                     valid[u] = m_raw < M && n < N;  // always true for the shapes this path runs
                     const uint32_t m = valid[u] ? (uint32_t)m_raw : 0u;
-                    const uint32_t X = m % out.nd, Y = (m / out.nd) % out.nd, Z = m / (out.nd * out.nd);
-                    cell[u] = (uint64_t)hcell0 + (uint64_t)(X * out.bx) +
-                              ((uint64_t)(Y * out.by) + (uint64_t)(Z * out.bz) * out.ny) * out.nx;
+                    cell[u] = hcell0 + (m & mnd) * sxb + ((m >> lnd) & mnd) * syb + (m >> (2u * lnd)) * szb;
                 }
                 if (!train) {
 #pragma unroll
                     for (int u = 0; u < 8; u++) {
                         fl[u] = out.flags[cell[u]];
-                        st[u] = stat[(size_t)(hc % 3) * out.ncell + cell[u]];
+                        st[u] = stat[cell[u]];
                     }
                 }
 #pragma unroll
@@ -279,9 +312,8 @@ __global__ void __launch_bounds__(256) k_gemm_nt_128(const __bf16* __restrict__ 
                     }
                     const float v = out.alpha * Cs[wave][t][lane];
                     if (!valid[u]) continue;
-                    if (train) out.fine[(size_t)hc * out.ncell + cell[u]] = v + far * (out.gain / in_s);
-                    else if ((fl[u] & 0x1F) != 0x01)
-                        dyn[(size_t)(hc % 3) * out.ncell + cell[u]] = (st[u] + in_s * sc * v) + kf * far;
+                    if (train) fine[cell[u]] = v + far * (out.gain / in_s);
+                    else if ((fl[u] & 0x1F) != 0x01) dyn[cell[u]] = (st[u] + in_s * sc * v) + kf * far;
                 }
             }
         } else {
@@ -618,10 +650,12 @@ hipError_t gemm_nt(int epi, const __bf16* A, const __bf16* B, int M, int N, int 
                    hipStream_t s) {
     if (K % BK != 0 || M <= 0 || N <= 0) return hipErrorInvalidValue;
     if (M >= BM2 && N >= BN2 && (epi != EPI_HEAD || (o.cpb & 63u) == 0)) {
-        const dim3 g((N + BN2 - 1) / BN2, (M + BM2 - 1) / BM2);
-        if (epi == EPI_F32) hipLaunchKernelGGL(k_gemm_nt_128<EPI_F32>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
-        else if (epi == EPI_BF16) hipLaunchKernelGGL(k_gemm_nt_128<EPI_BF16>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
-        else hipLaunchKernelGGL(k_gemm_nt_128<EPI_HEAD>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
+        const uint32_t MT = (uint32_t)(M + BM2 - 1) / BM2, NT = (uint32_t)(N + BN2 - 1) / BN2;
+        const int mi = NT > MT;
+        const dim3 g(mi ? XCDS * MT * ((NT + XCDS - 1) / XCDS) : MT * NT);
+        if (epi == EPI_F32) hipLaunchKernelGGL(k_gemm_nt_128<EPI_F32>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o, mi);
+        else if (epi == EPI_BF16) hipLaunchKernelGGL(k_gemm_nt_128<EPI_BF16>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o, mi);
+        else hipLaunchKernelGGL(k_gemm_nt_128<EPI_HEAD>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o, mi);
         return hipGetLastError();
     }
     const dim3 g((N + BN - 1) / BN, (M + BM - 1) / BM);
