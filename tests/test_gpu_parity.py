@@ -63,13 +63,16 @@ def test_exact_field_on_lod_tiles(shape, D):
     assert compare_all(gpu, orc, info) == []
 
 
-@pytest.mark.parametrize("float_type,trt,D", [(2, 0, 2), (1, 0, 2), (0, 1, 1), (2, 0, 0)])
-def test_mhd_subgrid_ecr(float_type, trt, D):
+@pytest.mark.parametrize("float_type,trt,D,inlet,graphics", [(2, 0, 2, False, True), (1, 0, 2, False, True),
+                                                              (0, 1, 1, False, True), (2, 0, 0, False, True),
+                                                              (1, 0, 2, True, True), (2, 1, 1, False, False)])
+def test_mhd_subgrid_ecr(float_type, trt, D, inlet, graphics):
     """SUBGRID_ECR (electron temperature, ECR heating by E_var, gyration
-    drift, ionisation), every buffer exact."""
-    rng = np.random.default_rng(700 + 10 * float_type + trt + D)
+    drift, ionisation) with TYPE_E cells, with the charge inlet, and with
+    field updates off (Et then not written), every buffer exact."""
+    rng = np.random.default_rng(700 + 10 * float_type + trt + D + 100 * inlet + 200 * (not graphics))
     gpu, orc, info = make_pair(rng, nx=16, ny=16, nz=16, mhd=True, lod_depth=D, float_type=float_type,
-                               relaxation_time=trt, ecr=True)
+                               relaxation_time=trt, ecr=True, charge_inlet=inlet, graphics=graphics)
     run_both(gpu, orc, 3)
     assert compare_all(gpu, orc, info) == []
 
