@@ -143,14 +143,15 @@ __global__ void __launch_bounds__(256) k_gemm_nt(const __bf16* __restrict__ A, c
 // multiplied out of LDS.
 //
 // Tile order (m_inner): when B has more tiles than A — the head, whose B
-// (weights, N = 6 x cells per LOD block rows) is 2 GB at 1024^3 against one
-// A row per LOD block — the blocks of one XCD walk all M tiles of one N tile
-// before the next. Workgroups go to the 8 XCDs round robin (block b runs on
-// XCD b % 8), so each B tile is fetched from HBM once per XCD and then served
-// by that XCD's L2, and A is re-read from the caches. Otherwise N tiles are
-// innermost.
+// (weights, N = 6 x cells per LOD block rows) is 2 GB at 1024^3 against
+// 5.5 MB of A (one row per LOD block) — workgroups go to the 8 XCDs round
+// robin (block b runs on XCD b % 8), and the blocks of one XCD walk its N
+// tiles for a group of GM2 M tiles (1.4 MB of A, which stays in the XCD's
+// 4 MB L2) before the next group: each B tile is fetched once per M group
+// and served from L2 to the group's other M tiles, and A is fetched once
+// per group instead of once per N tile. Otherwise N tiles are innermost.
 constexpr int BM2 = 128, BN2 = 128;
-constexpr uint32_t XCDS = 8;
+constexpr uint32_t XCDS = 8, GM2 = 8;
 
 template <int EPI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_gemm_nt_128(const __bf16* __restrict__ A, const __bf16* __restrict__ B, int M,
@@ -170,14 +171,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const uint32_t MT = (uint32_t)(M + BM2 - 1) / BM2, NT = (uint32_t)(N + BN2 - 1) / BN2;
     uint32_t tile_m, tile_n;
     if (m_inner) {
-        const uint32_t q = blockIdx.x / XCDS;
-        tile_m = q % MT;
-        tile_n = (q / MT) * XCDS + blockIdx.x % XCDS;
+        const uint32_t q = blockIdx.x / XCDS, NTG = (NT + XCDS - 1) / XCDS, r = q / GM2;
+        tile_m = (r / NTG) * GM2 + q % GM2;
+        tile_n = (r % NTG) * XCDS + blockIdx.x % XCDS;
     } else {
         tile_n = blockIdx.x % NT;
         tile_m = blockIdx.x / NT;
     }
-    if (tile_n >= NT) return;  // padding of the last group of N tiles (the whole block)
+    if (tile_n >= NT || tile_m >= MT) return;  // padding of the last tile groups (the whole block)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = (int)tile_m * BM2, n0 = (int)tile_n * BN2;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
@@ -289,9 +290,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 }
                 if (!train) {
 #pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        fl[u] = out.flags[cell[u]];
-                        st[u] = stat[cell[u]];
+                    for (int u = 0; u < 8; u++) {  // streamed once: nontemporal, so A stays in L2
+                        fl[u] = __builtin_nontemporal_load(out.flags + cell[u]);
+                        st[u] = __builtin_nontemporal_load(stat + cell[u]);
                     }
                 }
 #pragma unroll
@@ -312,8 +313,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                     }
                     const float v = out.alpha * Cs[wave][t][lane];
                     if (!valid[u]) continue;
-                    if (train) fine[cell[u]] = v + far * (out.gain / in_s);
-                    else if ((fl[u] & 0x1F) != 0x01) dyn[cell[u]] = (st[u] + in_s * sc * v) + kf * far;
+                    if (train) __builtin_nontemporal_store(v + far * (out.gain / in_s), fine + cell[u]);
+                    else if ((fl[u] & 0x1F) != 0x01)
+                        __builtin_nontemporal_store((st[u] + in_s * sc * v) + kf * far, dyn + cell[u]);
                 }
             }
         } else {
@@ -652,7 +654,7 @@ hipError_t gemm_nt(int epi, const __bf16* A, const __bf16* B, int M, int N, int 
     if (M >= BM2 && N >= BN2 && (epi != EPI_HEAD || (o.cpb & 63u) == 0)) {
         const uint32_t MT = (uint32_t)(M + BM2 - 1) / BM2, NT = (uint32_t)(N + BN2 - 1) / BN2;
         const int mi = NT > MT;
-        const dim3 g(mi ? XCDS * MT * ((NT + XCDS - 1) / XCDS) : MT * NT);
+        const dim3 g(mi ? XCDS * GM2 * ((NT + XCDS - 1) / XCDS) * ((MT + GM2 - 1) / GM2) : MT * NT);
         if (epi == EPI_F32) hipLaunchKernelGGL(k_gemm_nt_128<EPI_F32>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o, mi);
         else if (epi == EPI_BF16) hipLaunchKernelGGL(k_gemm_nt_128<EPI_BF16>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o, mi);
         else hipLaunchKernelGGL(k_gemm_nt_128<EPI_HEAD>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o, mi);
