@@ -179,7 +179,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         tile_m = blockIdx.x / NT;
     }
     if (tile_n >= NT || tile_m >= MT) return;  // padding of the last tile groups (the whole block)
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the wave index is uniform across the wave: scalar, so that the tile
+    // offsets, the head's component and the rows' cell bases stay scalar
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int m0 = (int)tile_m * BM2, n0 = (int)tile_n * BN2;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     f32x4 acc[4][4];
@@ -285,8 +287,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 for (int u = 0; u < 8; u++) {
                     const int m_raw = m0 + wm + i * 16 + tb + u;
                     valid[u] = m_raw < M && n < N;  // always true for the shapes this path runs
-                    const uint32_t m = valid[u] ? (uint32_t)m_raw : 0u;
-                    cell[u] = hcell0 + (m & mnd) * sxb + ((m >> lnd) & mnd) * syb + (m >> (2u * lnd)) * szb;
+                    // the row's block coordinates are wave-uniform (scalar); only
+                    // the lane's cell offset within the block varies
+                    const uint32_t m = m_raw < M ? (uint32_t)m_raw : 0u;
+                    cell[u] = hcell0 + (uint32_t)__builtin_amdgcn_readfirstlane(
+                                           (m & mnd) * sxb + ((m >> lnd) & mnd) * syb + (m >> (2u * lnd)) * szb);
                 }
                 if (!train) {
 #pragma unroll
