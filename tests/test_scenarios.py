@@ -45,3 +45,29 @@ def test_thruster_geometry():
     np.testing.assert_allclose(b[0, n // 2 + 5], -b[0, n // 2 - 5], rtol=1e-5, atol=1e-9)
     assert cfg.ext_equilibrium_boudaries and cfg.ext_magneto_hydro and cfg.ext_charge_inlet
     assert np.all(q[0][r < meta["tube_r"]] == np.float32(meta["q_inlet"]))
+
+
+def test_subgrid_ecr_workload_setup():
+    """add_subgrid_ecr (the mhd<N>-<fmt>-ecr bench workloads): the extension
+    on, E_var along x with the given magnitude, uniform initial Et, and the
+    arrays written to the E_VAR / ET buffers by apply()."""
+    from ionsolver_amd import Field
+    from ionsolver_amd.scenarios import add_subgrid_ecr, apply, mhd_blobs
+    cfg, arrays, _ = mhd_blobs(n=32, float_type=FloatType.FP16C, lod_depth=2, sigma=3.0, cyl_r=6.0)
+    cfg, arrays = add_subgrid_ecr(cfg, arrays, et0=2e-3, e_amp=5e-4)
+    n = 32 ** 3
+    assert cfg.ext_subgrid_ecr and cfg.ext_magneto_hydro
+    ev = arrays["e_var"].reshape(3, n)
+    assert np.all(ev[0] == np.float32(5e-4)) and not ev[1:].any()
+    assert arrays["et"].shape == (n,) and np.all(arrays["et"] == np.float32(2e-3))
+
+    class Sink:
+        def __init__(self):
+            self.seen = {}
+
+        def write_field(self, field, v):
+            self.seen[field] = v
+
+    s = Sink()
+    apply(s, arrays)
+    assert s.seen[Field.E_VAR] is arrays["e_var"] and s.seen[Field.ET] is arrays["et"]
