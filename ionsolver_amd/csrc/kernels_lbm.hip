@@ -223,6 +223,16 @@ __global__ void __launch_bounds__(MHD ? 512 : 256)
     if (active) {
         const bool is_e = a.eqb && bo == 0x02;
         const Nbr b = neighbours(n, a.nx, a.ny, a.nz);
+        // MHD: the neighbour offsets again before each store instead of nine
+        // registers held from the loads to the last store (the 32-bit MHD
+        // variants are at their 128-register limit); the empty asm keeps the
+        // compiler from reusing b
+        auto nb_store = [&]() -> Nbr {
+            if constexpr (!MHD) return b;
+            uint32_t m = n;
+            __asm__ volatile("" : "+v"(m));
+            return neighbours(m, a.nx, a.ny, a.nz);
+        };
         // every load of the cell first: fi, ei, fqi, E_dyn, B_dyn
         T rf[Q], re[MHD ? Q : 1], rq[MHD ? 7 : 1], rt[ECR ? 7 : 1];
         R Bx = R(0), By = R(0), Bz = R(0), Ex = R(0), Ey = R(0), Ez = R(0);
@@ -323,7 +333,7 @@ __global__ void __launch_bounds__(MHD ? 512 : 256)
             const R wq = WQ;
 #pragma unroll
             for (int i = 0; i < 7; i++) qhn[i] = inlet ? qeq[i] : fmaR(R(1) - wq, qhn[i], wq * qeq[i]);
-            store_ddfs<19, 7, S>((T*)a.fqi, a.N_store, n, b, a.odd, qhn);
+            store_ddfs<19, 7, S>((T*)a.fqi, a.N_store, n, nb_store(), a.odd, qhn);
             // electron gas part 2: F = -rho_e (E + u_e x B), Guo forcing, SRT
             const R crx = uyn_e * Bz - uzn_e * By;
             const R cry = uzn_e * Bx - uxn_e * Bz;
@@ -340,7 +350,7 @@ __global__ void __launch_bounds__(MHD ? 512 : 256)
                 const R fi = Fin[i] * c_tau;
                 ehn[i] = is_e ? feq[i] : fmaR(R(1) - w, ehn[i], fmaR(w, feq[i], fi));
             }
-            store_ddfs<Q, Q, S>((T*)a.ei, a.N_store, n, b, a.odd, ehn);
+            store_ddfs<Q, Q, S>((T*)a.ei, a.N_store, n, nb_store(), a.odd, ehn);
             // EM force on the gas (plain multiply-add, as written)
             fxn += rhon_q * (Ex + uyn * Bz - uzn * By);
             fyn += rhon_q * (Ey + uzn * Bx - uxn * Bz);
@@ -405,7 +415,7 @@ __global__ void __launch_bounds__(MHD ? 512 : 256)
                               : fmaR(R(0.5) * wp, feq[i] - fhn[i] + feb[i] - fhb[i],
                                                fmaR(R(0.5) * wm, feq[i] - feb[i] - fhn[i] + fhb[i], fhn[i] + Fin[i]));
         }
-        store_ddfs<Q, Q, S>((T*)a.fi, a.N_store, n, b, a.odd, fhn);
+        store_ddfs<Q, Q, S>((T*)a.fi, a.N_store, n, nb_store(), a.odd, fhn);
     }
 
     if (LOD) {
