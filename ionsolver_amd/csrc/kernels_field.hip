@@ -385,7 +385,8 @@ __global__ void __launch_bounds__(256) k_update_e_b_tiled(FieldArgs a, uint32_t 
     // so that every store (and E_stat / B_stat load) is contiguous.
     const uint64_t N = a.N;
     const uint32_t W = ROW * TX, cells = ROW * blockDim.x;
-    float* const lds = (float*)s_pq;  // cells floats fit in the source tile
+    static_assert(ROW * 256 <= FIELD_TILE * 4, "the epilogue stages ROW floats per work-item in s_pq");
+    float* const lds = (float*)s_pq;  // cells floats fit in the source tile (at most 256 work-items)
     const uint32_t own0 = x0 - bx * W + W * (ty + TY * tz);  // first own cell in block order
     uint32_t live = 0;  // bit j: cell threadIdx.x + j * blockDim.x is not solid
     uint64_t gn[ROW];
