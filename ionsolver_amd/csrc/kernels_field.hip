@@ -444,6 +444,28 @@ static uint32_t pow2_div(uint32_t v, uint32_t cap) {
     return p;
 }
 
+// Work-items per tile along x, y, z for LOD blocks of wx work-items (cells /
+// ROW) by ly by lz cells: powers of two where they give the most work-items,
+// else divisors with a product that is a multiple of 64 (whole wavefronts),
+// at most 16 along x and 256 in all; the larger x extent wins a tie.
+static void tile_shape(uint32_t wx, uint32_t ly, uint32_t lz, uint32_t& TX, uint32_t& TY, uint32_t& TZ) {
+    TX = pow2_div(wx, 16);
+    TY = pow2_div(ly, 256 / TX);
+    TZ = pow2_div(lz, 256 / (TX * TY));
+    uint32_t best = TX * TY * TZ;
+    for (uint32_t tx = 16; tx >= 1; tx--) {
+        if (wx % tx) continue;
+        for (uint32_t ty = 256 / tx; ty >= 1; ty--) {
+            if (ly % ty) continue;
+            for (uint32_t tz = 256 / (tx * ty); tz >= 1; tz--) {
+                const uint32_t p = tx * ty * tz;
+                if (lz % tz || p % 64u || p <= best) continue;
+                TX = tx; TY = ty; TZ = tz; best = p;
+            }
+        }
+    }
+}
+
 // Float64 sum of x over fluid cells (flags & 0x1F != 0x01): one partial per
 // block, summed on the host in block order.
 template <class T>
@@ -488,9 +510,8 @@ hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s) {
         const uint32_t lbx = a.nx / nd, lby = a.ny / nd, lbz = a.nz / nd;
         for (uint32_t row : {8u, 4u}) {
             if (lbx % row) continue;
-            const uint32_t TX = pow2_div(lbx / row, 16);
-            const uint32_t TY = pow2_div(lby, 256 / TX);
-            const uint32_t TZ = pow2_div(lbz, 256 / (TX * TY));
+            uint32_t TX, TY, TZ;
+            tile_shape(lbx / row, lby, lbz, TX, TY, TZ);
             if (TX * TY * TZ < 64) continue;
             const uint32_t blocks = (a.nx / (row * TX)) * (a.ny / TY) * (a.nz / TZ);
             const dim3 g(blocks), bl(TX * TY * TZ);

@@ -53,13 +53,15 @@ def test_mhd_rows_wider_than_a_block(nx, ny, nz, D, ft):
     assert compare_all(gpu, orc, info) == []
 
 
-@pytest.mark.parametrize("shape,D", [((64, 64, 64), 3), ((128, 64, 64), 3), ((64, 64, 32), 2), ((96, 64, 64), 2)])
+@pytest.mark.parametrize("shape,D", [((64, 64, 64), 3), ((128, 64, 64), 3), ((192, 64, 64), 3), ((64, 64, 32), 2),
+                                     ((96, 64, 64), 2)])
 def test_exact_field_on_lod_tiles(shape, D):
     """EXACT field arithmetic, every buffer exact against the oracle. At D = 3
     the near field is off (its block side n / 2^(2^D) is below one cell) and
-    the LOD blocks (8 and 16 cells along x) take the tiled kernel with 8 cells
-    per work-item (tiles of 1 x 8 x 8 and 2 x 8 x 8 work-items); at D = 2 these
-    lattices have a near field, which the generic kernel evaluates."""
+    the LOD blocks (8, 16 and 24 cells along x) take the tiled kernel with 8
+    cells per work-item (tiles of 1 x 8 x 8, 2 x 8 x 8 and 3 x 8 x 8
+    work-items); at D = 2 these lattices have a near field, which the generic
+    kernel evaluates."""
     rng = np.random.default_rng(600 + shape[0])
     gpu, orc, info = make_pair(rng, nx=shape[0], ny=shape[1], nz=shape[2], mhd=True, lod_depth=D, float_type=2)
     run_both(gpu, orc, 2)
@@ -183,7 +185,8 @@ def test_save_load_roundtrip(tmp_path):
     assert int(back.config.float_type) == 1 and back.config.n_x == 16 and back.config.ext_magneto_hydro
 
 
-@pytest.mark.parametrize("shape,D", [((64, 64, 64), 3), ((128, 64, 64), 3), ((64, 64, 32), 2), ((32, 32, 32), 3)])
+@pytest.mark.parametrize("shape,D", [((64, 64, 64), 3), ((128, 64, 64), 3), ((192, 64, 64), 3), ((64, 64, 32), 2),
+                                     ((32, 32, 32), 3)])
 def test_fast_field_vs_exact_one_step(shape, D):
     """FAST vs EXACT field arithmetic from identical LOD sums (the LOD-tiled
     kernel serves the D = 3 shapes of 64 cells and more per axis, the generic
