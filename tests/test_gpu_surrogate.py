@@ -72,10 +72,12 @@ def test_initial_head_is_the_direct_near_field():
     assert rel_l2(b_sur, b_dir) < 1e-2
 
 
-@pytest.mark.parametrize("n,solid_frac", [(16, 0.0), (16, 0.2), (32, 0.1)])
+@pytest.mark.parametrize("n,solid_frac", [(16, 0.0), (16, 0.2), (32, 0.1), (64, 0.1)])
 def test_forward_loss_and_gradients_vs_float64_reference(n, solid_frac):
     """16^3: every GEMM on the 64 x 64 kernel; 32^3: the head and the large
-    input-gradient GEMMs on the 128 x 128 kernel."""
+    input-gradient GEMMs on the 128 x 128 kernel, N tiles innermost; 64^3:
+    the head with more N tiles (24) than M tiles (4), in the XCD-grouped
+    order whose M group of 8 tiles is half padding."""
     lbm = make_lbm(n, 3, solid_frac=solid_frac, seed=3)
     geo = S.Geometry(n, n, n, 3, 8)
     rng = np.random.default_rng(4)
