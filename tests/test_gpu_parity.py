@@ -53,15 +53,16 @@ def test_mhd_rows_wider_than_a_block(nx, ny, nz, D, ft):
     assert compare_all(gpu, orc, info) == []
 
 
-@pytest.mark.parametrize("shape,D", [((64, 64, 64), 3), ((128, 64, 64), 3), ((192, 64, 64), 3), ((64, 64, 32), 2),
-                                     ((96, 64, 64), 2)])
+@pytest.mark.parametrize("shape,D", [((64, 64, 64), 3), ((128, 64, 64), 3), ((192, 64, 64), 3), ((32, 64, 64), 3),
+                                     ((64, 64, 32), 2), ((96, 64, 64), 2)])
 def test_exact_field_on_lod_tiles(shape, D):
     """EXACT field arithmetic, every buffer exact against the oracle. At D = 3
     the near field is off (its block side n / 2^(2^D) is below one cell) and
     the LOD blocks (8, 16 and 24 cells along x) take the tiled kernel with 8
     cells per work-item (tiles of 1 x 8 x 8, 2 x 8 x 8 and 3 x 8 x 8
-    work-items); at D = 2 these lattices have a near field, which the generic
-    kernel evaluates."""
+    work-items), 4-cell blocks the one with 4 cells per work-item (1 x 8 x 8);
+    at D = 2 these lattices have a near field, which the generic kernel
+    evaluates."""
     rng = np.random.default_rng(600 + shape[0])
     gpu, orc, info = make_pair(rng, nx=shape[0], ny=shape[1], nz=shape[2], mhd=True, lod_depth=D, float_type=2)
     run_both(gpu, orc, 2)
