@@ -187,7 +187,8 @@ __global__ void __launch_bounds__(256) k_initialize(LbmArgs a) {
 // summed per segment by the pairwise tree in LDS and written to lod_seg.
 // ---------------------------------------------------------------------------
 // MHD variants with 32-bit arithmetic are held to 128 VGPRs (4 wavefronts per
-// SIMD): the 16-bit formats otherwise use a few more and drop to 3.
+// SIMD); with the neighbour offsets recomputed before the stores they need
+// 111-114, and holding them to 96 (5 wavefronts) spills 14-20.
 template <int Q, class S, bool MHD, bool TRT, bool LOD, bool ECR = false>
 __global__ void __launch_bounds__(MHD ? 512 : 256)
     __attribute__((amdgpu_waves_per_eu(MHD && !ECR && sizeof(typename S::R) == 4 ? 4 : 1))) k_stream_collide(LbmArgs a) {

@@ -153,13 +153,15 @@ __global__ void __launch_bounds__(256) k_gemm_nt(const __bf16* __restrict__ A, c
 constexpr int BM2 = 128, BN2 = 128;
 constexpr uint32_t XCDS = 8, GM2 = 8;
 
+// Held to 128 VGPRs (the accumulators in VGPRs, not AGPRs): 4 waves per SIMD.
 template <int EPI>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_gemm_nt_128(const __bf16* __restrict__ A, const __bf16* __restrict__ B, int M,
-                                                     int N, int K, int lda, int ldb, GemmOut out, int m_inner) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+    k_gemm_nt_128(const __bf16* __restrict__ A, const __bf16* __restrict__ B, int M, int N, int K, int lda, int ldb,
+                  GemmOut out, int m_inner) {
     // The K-loop tiles (As, Bs) and the epilogue's staging (Cs, Fs) share one
     // LDS allocation: the epilogue's first barrier follows the last MFMA
-    // reads. 24.3 KB instead of 45 KB per block, so twice as many blocks fit
-    // on a CU.
+    // reads. 24.3 KB instead of 45 KB per block, so LDS no longer limits the
+    // blocks per CU (registers allow 4).
     typedef __bf16 Tile[BM2][BK + LDS_PAD];
     typedef float CTile[4][16][64 + 4];
     typedef float FTile[4][16][FAR_PTS];
