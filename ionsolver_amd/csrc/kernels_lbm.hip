@@ -62,8 +62,9 @@ template <class R> __device__ __forceinline__ typename V4<R>::T mk4(R x, R y, R 
 // which gives the global load/store form with a scalar 64-bit base and a
 // 32-bit vector offset (one VALU shift per distinct index instead of 64-bit
 // address arithmetic per access). Valid while idx * sizeof(T) < 2^32: every
-// array here has at most N <= 2^30 elements of at most 4 bytes per base
-// (the host rejects larger lattices).
+// array here has at most N elements per base, and the host rejects lattices
+// with N > 2^30 (elements of at most 4 bytes) and, for the FP64
+// instantiation (8-byte elements), N > 2^29.
 template <class T> __device__ __forceinline__ T* at(T* base, uint32_t idx) {
     return (T*)((char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
 }

@@ -370,6 +370,9 @@ int validate(const ion_config& c) {
     if (N > 0x40000000ull)
         return fail(ION_ERR_UNSUPPORTED, "more than 2^30 cells (1024^3) on one GPU: the kernels address every "
                                          "per-direction array with 32-bit byte offsets");
+    if (c.float_type == ION_FP64 && N > 0x20000000ull)
+        return fail(ION_ERR_UNSUPPORTED, "more than 2^29 cells with FP64: the kernels address every per-direction "
+                                         "array with 32-bit byte offsets, and 8-byte elements reach 2^32 bytes there");
     if (c.ext_subgrid_ecr) {
         if (!c.ext_magneto_hydro) return fail(ION_ERR_INVALID, "ext_subgrid_ecr needs ext_magneto_hydro");
         if (c.velocity_set != ION_D3Q19 || c.float_type == ION_FP64)

@@ -103,6 +103,8 @@ def base_config(**kw):
     (dict(ext_subgrid_ecr=True, ext_magneto_hydro=True, ext_volume_force=True,
           velocity_set=I.VelocitySet.D3Q15), -2),             # built for D3Q19
     (dict(ext_force_field=True), -1),
+    (dict(n_x=1024, n_y=1024, n_z=1025, float_type=I.FloatType.FP32), -2),  # 2^30 + 2^20 cells
+    (dict(n_x=1024, n_y=1024, n_z=513, float_type=I.FloatType.FP64), -2),  # FP64 above 2^29 cells
 ])
 def test_invalid_configs_are_rejected_before_touching_the_device(kw, code):
     with pytest.raises(I.IonError) as e:
