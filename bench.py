@@ -7,9 +7,16 @@ update_e_b_dynamic field step, over the whole lattice.
 
 Default workload (the configuration the headline metric is quoted on):
 256^3 D3Q19 SRT, FP32 DDFs, MHD with 8 seeded Gaussian charge blobs
-(sigma = 6 cells), uniform B0 plus a point dipole, Guo Lorentz forcing, a
-solid cylinder r = 40, LOD depth 4. Inputs are resident in HBM before the
-timed region. Prints one JSON line.
+(sigma = 6 cells) on a neutral plasma background (ion charge density 1 +
+blobs, electron gas started at density 1, plasma units: KE 1.0e-3,
+KMU 2.8e-4, KKGE -2.5), uniform B0 plus a point dipole, Guo Lorentz forcing,
+a solid cylinder r = 40, LOD depth 4. A workload name ending in "-residue"
+starts the electron gas at density 0 in the blob workloads' original units,
+as the reference's initialize does; that state turns NaN within about ten
+steps. Inputs are resident in HBM before the timed region. Prints one JSON
+line: the default workload, beside it the same workload with FP16C storage,
+the state's finiteness after the timed steps and its error against the
+FP64 instantiation of the same kernels at 128^3.
 """
 import argparse
 import json
@@ -60,17 +67,19 @@ def bytes_per_cell(cfg):
 def workload(name):
     from ionsolver_amd import FloatType, scenarios
     ft = {"fp32": FloatType.FP32, "fp16s": FloatType.FP16S, "fp16c": FloatType.FP16C}
+    plasma = None if name.endswith("-residue") else 1.0
     if name.startswith("mhd"):
         parts = name.split("-")
         n = int(parts[0][3:])
-        fmt = parts[1] if len(parts) > 1 else "fp32"
-        cfg, arrays, blobs = scenarios.mhd_blobs(n=n, float_type=ft[fmt])
-        if "ecr" in parts[2:]:
+        fmt = parts[1] if len(parts) > 1 and parts[1] in ft else "fp32"
+        cfg, arrays, blobs = scenarios.mhd_blobs(n=n, float_type=ft[fmt], plasma=plasma)
+        if "ecr" in parts[1:]:
             scenarios.add_subgrid_ecr(cfg, arrays)
-        return cfg, arrays, dict(blobs=len(blobs), seed=1234)
+        return cfg, arrays, dict(blobs=len(blobs), seed=1234, plasma=plasma)
     if name.startswith("thruster"):
         n = int(name[8:].split("-")[0])
-        cfg, arrays, meta = scenarios.thruster(n=n, log=lambda m: print(m, file=sys.stderr, flush=True))
+        cfg, arrays, meta = scenarios.thruster(n=n, log=lambda m: print(m, file=sys.stderr, flush=True),
+                                               plasma=plasma)
         return cfg, arrays, meta
     if name.startswith("tgv"):
         parts = name.split("-")
@@ -90,7 +99,8 @@ def cpu_baseline(name, seconds_hint):
     from ionsolver_amd import scenarios
     if name.startswith("mhd"):
         n_sample = 128 if int(name.split("-")[0][3:]) >= 128 else int(name.split("-")[0][3:])
-        cfg, arrays, _ = scenarios.mhd_blobs(n=n_sample, float_type=workload(name)[0].float_type)
+        cfg, arrays, _ = scenarios.mhd_blobs(n=n_sample, float_type=workload(name)[0].float_type,
+                                             plasma=None if name.endswith("-residue") else 1.0)
         steps = 1
     else:
         n_sample = int(name.split("-")[0][3:])
@@ -219,39 +229,47 @@ def rel_l2(a, b):
     return float(np.sqrt(np.sum((a - b) ** 2)) / d) if d > 0 else float(np.sqrt(np.sum(a * a)))
 
 
-def format_errors(n_mhd=128, mhd_steps=3, nan_steps=24, n_tgv=128, tgv_steps=1000):
+def format_errors(n_mhd=128, mhd_steps=200, nan_steps=24, n_tgv=128, tgv_steps=1000):
     """Relative L2 error of each DDF storage format (FP32 arithmetic) against
     the FP64 instantiation of the same kernels on the same GPU: the MHD blob
-    workload at n_mhd^3 (EXACT field arithmetic) after mhd_steps steps, the
-    first step with a NaN within nan_steps steps (the electron-gas knife-edge
-    of the reference's initial state), and the Taylor-Green vortex at n_tgv^3
-    after tgv_steps steps."""
+    workload on its plasma background at n_mhd^3 (EXACT field arithmetic)
+    after mhd_steps steps, the reference's initial state (electron gas at
+    density 0) after 3 steps and the first step with a NaN there within
+    nan_steps steps (its electron-gas knife-edge), and the Taylor-Green vortex
+    at n_tgv^3 after tgv_steps steps."""
     import ionsolver_amd as I
     from ionsolver_amd import Field, FieldMode, FloatType, scenarios
     fmts = [FloatType.FP64, FloatType.FP32, FloatType.FP16S, FloatType.FP16C]
-    mhd, first_nan = {}, {}
-    for ft in fmts:
-        cfg, arrays, _ = scenarios.mhd_blobs(n=n_mhd, float_type=ft, lod_depth=4)
-        lbm = I.Lbm(cfg)
-        scenarios.apply(lbm, arrays)
-        lbm.set_field_mode(FieldMode.EXACT)
-        lbm.initialize()
-        lbm.run(mhd_steps)
-        mhd[ft] = {k: lbm.read_field(f).astype(np.float64) for k, f in
-                   (("rho", Field.RHO), ("u", Field.U), ("E", Field.E_DYN), ("B", Field.B_DYN), ("Q", Field.Q))}
-        first_nan[ft.name] = None
-        for t in range(mhd_steps, nan_steps):
-            if np.isnan(lbm.read_field(Field.Q)).any():
-                first_nan[ft.name] = t
-                break
-            lbm.do_time_step()
-        lbm.close()
-    ref = mhd[FloatType.FP64]
-    table = {}
-    for ft in fmts[1:]:
-        table[ft.name] = {"rho-1": rel_l2(mhd[ft]["rho"] - 1, ref["rho"] - 1), "u": rel_l2(mhd[ft]["u"], ref["u"]),
-                          "E": rel_l2(mhd[ft]["E"], ref["E"]), "B": rel_l2(mhd[ft]["B"], ref["B"]),
-                          "Q": rel_l2(mhd[ft]["Q"], ref["Q"])}
+    tables, first_nan = {}, {}
+    for plasma, steps in ((1.0, mhd_steps), (None, 3)):
+        mhd = {}
+        for ft in fmts:
+            cfg, arrays, _ = scenarios.mhd_blobs(n=n_mhd, float_type=ft, lod_depth=4, plasma=plasma)
+            lbm = I.Lbm(cfg)
+            scenarios.apply(lbm, arrays)
+            lbm.set_field_mode(FieldMode.EXACT)
+            lbm.initialize()
+            lbm.run(steps)
+            mhd[ft] = {k: lbm.read_field(f).astype(np.float64) for k, f in
+                       (("rho", Field.RHO), ("u", Field.U), ("E", Field.E_DYN), ("B", Field.B_DYN), ("Q", Field.Q),
+                        ("Es", Field.E_STAT), ("Bs", Field.B_STAT))}
+            if plasma is None:
+                first_nan[ft.name] = None
+                for t in range(steps, nan_steps):
+                    if np.isnan(lbm.read_field(Field.Q)).any():
+                        first_nan[ft.name] = t
+                        break
+                    lbm.do_time_step()
+            lbm.close()
+        ref = mhd[FloatType.FP64]
+        table = {}
+        for ft in fmts[1:]:
+            g = mhd[ft]
+            table[ft.name] = {"rho-1": rel_l2(g["rho"] - 1, ref["rho"] - 1), "u": rel_l2(g["u"], ref["u"]),
+                              "E-E_stat": rel_l2(g["E"] - g["Es"], ref["E"] - ref["Es"]),
+                              "B-B_stat": rel_l2(g["B"] - g["Bs"], ref["B"] - ref["Bs"]),
+                              "Q": rel_l2(g["Q"], ref["Q"])}
+        tables["plasma" if plasma else "residue"] = (steps, table)
     tgv = {}
     for ft in fmts:
         cfg, arrays = scenarios.fluid_tgv(n=n_tgv, float_type=ft)
@@ -264,9 +282,174 @@ def format_errors(n_mhd=128, mhd_steps=3, nan_steps=24, n_tgv=128, tgv_steps=100
     tgv_table = {ft.name: {"rho-1": rel_l2(tgv[ft][0], tgv[FloatType.FP64][0]),
                            "u": rel_l2(tgv[ft][1], tgv[FloatType.FP64][1])} for ft in fmts[1:]}
     return {"metric": "storage-format rel-L2 vs the FP64 instantiation", "unit": "relative L2",
-            "mhd": {"workload": f"mhd{n_mhd} blobs, D=4, EXACT fields", "steps": mhd_steps, "errors": table,
-                    "first_nan_step": first_nan, "nan_window": nan_steps},
+            "mhd": {"workload": f"mhd{n_mhd} blobs on the plasma background, D=4, EXACT fields",
+                    "steps": tables["plasma"][0], "errors": tables["plasma"][1]},
+            "mhd_residue": {"workload": f"mhd{n_mhd} blobs from the reference's initial state, D=4, EXACT fields",
+                            "steps": tables["residue"][0], "errors": tables["residue"][1],
+                            "first_nan_step": first_nan, "nan_window": nan_steps},
             "tgv": {"workload": f"tgv{n_tgv}", "steps": tgv_steps, "errors": tgv_table}}
+
+
+def state_check(lbm, cfg):
+    """Finiteness and ranges of the state after the timed steps (read after
+    the timed region)."""
+    from ionsolver_amd import Field
+    u = lbm.read_field(Field.U)
+    rho = lbm.read_field(Field.RHO)
+    out = {"finite": bool(np.isfinite(u).all() and np.isfinite(rho).all()),
+           "u_max": float(np.nanmax(np.abs(u))) if np.isfinite(u).any() else None}
+    if cfg.ext_magneto_hydro:
+        q = lbm.read_field(Field.Q)
+        e = lbm.read_field(Field.E_DYN)
+        out["finite"] = bool(out["finite"] and np.isfinite(q).all() and np.isfinite(e).all())
+        out["q_abs_max"] = float(np.nanmax(np.abs(q))) if np.isfinite(q).any() else None
+    return out
+
+
+def fp64_error(name, steps, field_mode, n=128):
+    """Relative L2 error of the measured configuration (same workload, storage
+    format and field arithmetic, same number of steps) against the FP64
+    instantiation of the same kernels with EXACT field arithmetic, at n^3 (the
+    FP64 field kernel is the generic one, so n = 128 keeps it to seconds)."""
+    import ionsolver_amd as I
+    from ionsolver_amd import Field, FieldMode, FloatType, scenarios
+    cfg0, _, _ = workload(name)
+    res = {}
+    for ft, mode in ((FloatType.FP64, FieldMode.EXACT), (cfg0.float_type, field_mode)):
+        if name.startswith("mhd"):
+            cfg, arrays, _ = scenarios.mhd_blobs(n=n, float_type=ft, plasma=None if name.endswith("-residue") else 1.0)
+        else:
+            cfg, arrays = scenarios.fluid_tgv(n=n, float_type=ft)
+        lbm = I.Lbm(cfg)
+        try:
+            scenarios.apply(lbm, arrays)
+            lbm.set_field_mode(mode)
+            lbm.initialize()
+            lbm.run(steps)
+            r = {"rho-1": lbm.read_field(Field.RHO).astype(np.float64) - 1.0,
+                 "u": lbm.read_field(Field.U).astype(np.float64)}
+            if cfg.ext_magneto_hydro:
+                r["E"] = lbm.read_field(Field.E_DYN).astype(np.float64) - lbm.read_field(Field.E_STAT)
+                r["B"] = lbm.read_field(Field.B_DYN).astype(np.float64) - lbm.read_field(Field.B_STAT)
+                r["Q"] = lbm.read_field(Field.Q).astype(np.float64)
+            res[ft] = r
+        finally:
+            lbm.close()
+    ref, got = res[FloatType.FP64], res[cfg0.float_type]
+    return {"lattice": [n, n, n], "steps": steps, "storage": cfg0.float_type.name,
+            "field_mode": "fast" if field_mode == FieldMode.FAST else "exact",
+            "reference": "FP64 instantiation (double DDFs, fields, arithmetic; EXACT fields)",
+            "rel_l2": {k: rel_l2(got[k], ref[k]) for k in ref}}
+
+
+def measure(name, steps, warmup, field_mode, thruster=False):
+    """Set up a workload, run `warmup` steps, time `steps` steps (host clock
+    around run(), per-kernel HIP events on the library stream). Returns the
+    open Lbm and the measurements."""
+    import ionsolver_amd as I
+    from ionsolver_amd import scenarios
+    cfg, arrays, meta = workload(name)
+    lbm = I.Lbm(cfg)
+    scenarios.apply(lbm, arrays)
+    flags = arrays.get("flags")
+    del arrays
+    lbm.set_field_mode(field_mode)
+    if thruster:
+        # learned field step, the direct kernel every 50 field steps as judge
+        lbm.surrogate_create(channels=16)
+        lbm.set_field_solver(I.FieldSolver.SURROGATE, judge_every=50, tolerance=0.01)
+    lbm.initialize()
+    if warmup:
+        lbm.run(warmup)
+    lbm.set_timing(True)
+    lbm.synchronize()
+    t0 = time.perf_counter()
+    done, shown = 0, t0
+    while done < steps:  # in chunks of up to 25 steps, a progress line at least every 30 s
+        k_run = min(25, steps - done)
+        lbm.run(k_run)  # enqueues the steps, synchronises at the end
+        done += k_run
+        now = time.perf_counter()
+        if now - shown > 30.0:
+            print(f"step {done}/{steps}: {now - t0:.1f} s", file=sys.stderr, flush=True)
+            shown = now
+    dt = time.perf_counter() - t0
+    tm = lbm.get_timing()
+    lbm.set_timing(False)
+    N = cfg.n_x * cfg.n_y * cfg.n_z
+    n_f = int(np.sum((flags & 0x1F) != 1)) if flags is not None else N
+    sc_ms = tm["ms_stream_collide"] / steps
+    sc_bytes = n_f * bytes_per_cell(cfg) + (N - n_f) * 1
+    sc = {"ms": round(sc_ms, 4), "bytes": sc_bytes,
+          "GB/s": round(sc_bytes / sc_ms / 1e6, 1) if sc_ms else None,
+          "frac_hbm": round(sc_bytes / sc_ms / 1e6 / HBM_PEAK_GBS, 4) if sc_ms else None}
+    return lbm, dict(cfg=cfg, meta=meta, flags=flags, N=N, n_f=n_f, dt=dt, tm=tm,
+                     mlups=N * steps / dt / 1e6, sc=sc)
+
+
+def field_interactions(cfg, flags):
+    """Interactions of one direct field step: fluid cells x top-level LOD
+    sources inside the range the reference reads, less each cell's own LOD."""
+    D = cfg.mhd_lod_depth
+    nd = 1 << D
+    n_top = nd ** 3
+    lo = ((1 << (3 * (D + 1))) - 1) // 7 - n_top
+    N = cfg.n_x * cfg.n_y * cfg.n_z
+    z, rem = np.divmod(np.arange(N, dtype=np.int64), cfg.n_x * cfg.n_y)
+    y, x = np.divmod(rem, cfg.n_x)
+    own = x // (cfg.n_x // nd) + (y // (cfg.n_y // nd) + z // (cfg.n_z // nd) * nd) * nd
+    fl = (flags & 0x1F) != 1
+    return int(np.count_nonzero(fl)) * (n_top - lo) - int(np.sum(fl & (own >= lo)))
+
+
+def probes(out, kernels, cfg, N):
+    """HBM ceilings beside stream_collide: a plain copy, the kernel's access
+    shape at the cell, and its in-place esoteric-pull addressing."""
+    import ionsolver_amd as I
+    _, copy_gbs = I.hbm_copy_probe(1 << 30, 10)
+    out["hbm_copy_probe"] = {"GB/s": round(copy_gbs, 1), "frac_of_peak": round(copy_gbs / HBM_PEAK_GBS, 4),
+                             "note": "best of 10 nontemporal 16-byte copies of 1 GiB (read + write)"}
+    sc = kernels["stream_collide"]
+    sc["frac_of_copy_probe"] = round(sc["GB/s"] / copy_gbs, 4) if sc["GB/s"] else None
+    elem = 4 if int(cfg.float_type) == 2 else 2
+    streams = 45 if cfg.ext_magneto_hydro else 19
+    cells = min(N, 1 << 24)
+    _, pat_gbs = I.hbm_stream_probe(elem, streams, cells, 10)
+    out["pattern_probe"] = {"GB/s": round(pat_gbs, 1), "frac_of_peak": round(pat_gbs / HBM_PEAK_GBS, 4),
+                            "streams": streams, "bytes_per_element": elem, "cells": cells,
+                            "note": "one work-item per cell loading its SoA stream elements then storing them to "
+                                    "a second buffer, nontemporal, best of 10 (read + write)"}
+    sc["frac_of_pattern_probe"] = round(sc["GB/s"] / pat_gbs, 4) if sc["GB/s"] else None
+    if N <= (1 << 27) and cfg.velocity_set.name == "D3Q19":
+        pull_gbs = pull_probe_gbs(elem, streams, cfg)
+        out["pull_probe"] = {"GB/s": round(pull_gbs, 1), "frac_of_peak": round(pull_gbs / HBM_PEAK_GBS, 4),
+                             "streams": streams, "bytes_per_element": elem, "lattice": [cfg.n_x, cfg.n_y, cfg.n_z],
+                             "note": "one work-item per cell reading every SoA array at the cell or at n + c_i "
+                                     "(esoteric pull, even step), then rewriting the same slots, nontemporal, "
+                                     "best of 10 (read + write)"}
+        sc["frac_of_pull_probe"] = round(sc["GB/s"] / pull_gbs, 4) if sc["GB/s"] else None
+
+
+def pull_probe_gbs(elem, streams, cfg):
+    import ionsolver_amd as I
+    return I.hbm_pull_probe(elem, streams, cfg.n_x, cfg.n_y, cfg.n_z, 10)[1]
+
+
+def fp16c_leg(name, steps, warmup, field_mode):
+    """The same MHD workload with FP16C DDF storage (FP32 arithmetic): the
+    storage format of the HBM target, timed like the main line."""
+    lbm, m = measure(name, steps, warmup, field_mode)
+    try:
+        state = state_check(lbm, m["cfg"])
+    finally:
+        lbm.close()
+    sc = m["sc"]
+    pull = pull_probe_gbs(2, 45, m["cfg"])
+    sc["frac_of_pull_probe"] = round(sc["GB/s"] / pull, 4) if sc["GB/s"] else None
+    return {"workload": name, "value": round(m["mlups"], 3), "unit": "MLUPs/s", "steps": steps, "warmup": warmup,
+            "ms_per_step": round(m["dt"] / steps * 1e3, 4), "stream_collide": sc,
+            "update_e_b_dynamic_ms": round(m["tm"]["ms_update_e_b"] / steps, 4), "state": state,
+            "pull_probe_GB/s": round(pull, 1)}
 
 
 def main():
@@ -280,6 +463,9 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned field surrogate leg")
     ap.add_argument("--surrogate-train", type=int, default=2000,
                     help="GPU training steps (fresh snapshots) of the surrogate leg")
+    ap.add_argument("--no-fp16c", action="store_true", help="skip the FP16C storage leg of an FP32 MHD workload")
+    ap.add_argument("--no-fp64-error", action="store_true",
+                    help="skip the error of the measured configuration against the FP64 instantiation")
     ap.add_argument("--format-errors", action="store_true",
                     help="print the storage-format error table against the FP64 instantiation instead")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -296,7 +482,7 @@ def main():
         sys.exit(2)
 
     import ionsolver_amd as I
-    from ionsolver_amd import FieldMode, scenarios
+    from ionsolver_amd import FieldMode
 
     if args.format_errors:
         print(json.dumps(format_errors()), flush=True)
@@ -306,64 +492,19 @@ def main():
         return
     if args.calibrate:
         I.hbm_copy_probe(1 << 30, 2)
-    cfg, arrays, meta = workload(args.workload)
-    lbm = I.Lbm(cfg)
-    scenarios.apply(lbm, arrays)
-    flags = arrays.get("flags")
-    del arrays
-    lbm.set_field_mode(FieldMode.FAST if args.field_mode == "fast" else FieldMode.EXACT)
     thruster = args.workload.startswith("thruster")
-    if thruster:
-        # learned field step, the direct kernel every 50 field steps as judge
-        lbm.surrogate_create(channels=16)
-        lbm.set_field_solver(I.FieldSolver.SURROGATE, judge_every=50, tolerance=0.01)
-    lbm.initialize()
-    if args.warmup:
-        lbm.run(args.warmup)
-    lbm.set_timing(True)
-    lbm.synchronize()
-    t0 = time.perf_counter()
-    done, shown = 0, t0
-    while done < args.steps:  # in chunks of up to 25 steps, a progress line at least every 30 s
-        k_run = min(25, args.steps - done)
-        lbm.run(k_run)  # enqueues the steps, synchronises at the end
-        done += k_run
-        now = time.perf_counter()
-        if now - shown > 30.0:
-            print(f"step {done}/{args.steps}: {now - t0:.1f} s", file=sys.stderr, flush=True)
-            shown = now
-    dt = time.perf_counter() - t0
-    tm = lbm.get_timing()
-    lbm.set_timing(False)
-
-    N = cfg.n_x * cfg.n_y * cfg.n_z
-    n_f = int(np.sum((flags & 0x1F) != 1)) if flags is not None else N
-    mlups = N * args.steps / dt / 1e6
-    k = args.steps
-    sc_ms = tm["ms_stream_collide"] / k
-    sc_bytes = n_f * bytes_per_cell(cfg) + (N - n_f) * 1
-    kernels = {"stream_collide": {"ms": round(sc_ms, 4), "bytes": sc_bytes,
-                                  "GB/s": round(sc_bytes / sc_ms / 1e6, 1) if sc_ms else None,
-                                  "frac_hbm": round(sc_bytes / sc_ms / 1e6 / HBM_PEAK_GBS, 4) if sc_ms else None}}
-    roofline = None
+    mode = FieldMode.FAST if args.field_mode == "fast" else FieldMode.EXACT
+    lbm, m = measure(args.workload, args.steps, args.warmup, mode, thruster=thruster)
+    cfg, tm, k, N, n_f = m["cfg"], m["tm"], args.steps, m["N"], m["n_f"]
+    state = state_check(lbm, cfg)
+    kernels = {"stream_collide": m["sc"]}
     if thruster:
         kernels["field_step"] = {"ms": round(tm["ms_update_e_b"] / k, 4), "solver": "surrogate, judged every 50"}
         kernels["lod_reduce"] = {"ms": round(tm["ms_lod_reduce"] / k, 4)}
-        roofline = {"bound": "hbm", "achieved": kernels["stream_collide"]["GB/s"], "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": kernels["stream_collide"]["frac_hbm"], "traffic": None,
-                    "kernel": "stream_collide"}
+        roofline = {"bound": "hbm", "achieved": m["sc"]["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": m["sc"]["frac_hbm"], "traffic": None, "kernel": "stream_collide"}
     elif cfg.ext_magneto_hydro:
-        D = cfg.mhd_lod_depth
-        nd = 1 << D
-        n_top = nd ** 3
-        lo = ((1 << (3 * (D + 1))) - 1) // 7 - n_top
-        # own LOD index of every fluid cell; interactions = N_f*(sources in range) - #(own LOD in range)
-        z, rem = np.divmod(np.arange(N, dtype=np.int64), cfg.n_x * cfg.n_y)
-        y, x = np.divmod(rem, cfg.n_x)
-        own = x // (cfg.n_x // nd) + (y // (cfg.n_y // nd) + z // (cfg.n_z // nd) * nd) * nd
-        fl = (flags & 0x1F) != 1
-        in_range = n_top - lo  # top-level LODs inside [lo, n_lod_own)
-        interactions = n_f * in_range - int(np.sum(fl & (own >= lo)))
+        interactions = field_interactions(cfg, m["flags"])
         eb_ms = tm["ms_update_e_b"] / k
         eb_tflops = interactions * FLOP_PER_INTERACTION / (eb_ms * 1e-3) / 1e12
         kernels["update_e_b_dynamic"] = {"ms": round(eb_ms, 4), "interactions": interactions,
@@ -371,55 +512,29 @@ def main():
         kernels["lod_reduce"] = {"ms": round(tm["ms_lod_reduce"] / k, 4)}
         kernels["field_prep"] = {"ms": round(tm["ms_field_prep"] / k, 4)}
         kernels["clear_qu_lod"] = {"ms": round(tm["ms_other"] / k, 4)}
-        roofline = {"bound": "mfma", "achieved": round(eb_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        # the direct field kernel runs on the FP32 vector ALUs: its bound is
+        # VALU issue, priced against MI355X's published FP32 vector peak
+        roofline = {"bound": "valu", "achieved": round(eb_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(eb_tflops / FP32_PEAK_TFLOPS, 4),
                     "traffic": PMC_TRAFFIC.get((args.workload, "update_e_b_dynamic")),
                     "kernel": "update_e_b_dynamic", "compute": "fp32 VALU"}
     else:
-        roofline = {"bound": "hbm", "achieved": kernels["stream_collide"]["GB/s"], "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": kernels["stream_collide"]["frac_hbm"],
-                    "traffic": PMC_TRAFFIC.get((args.workload, "stream_collide")), "kernel": "stream_collide"}
-    kernels["stream_collide"]["traffic"] = PMC_TRAFFIC.get((args.workload, "stream_collide"))
+        roofline = {"bound": "hbm", "achieved": m["sc"]["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": m["sc"]["frac_hbm"], "traffic": PMC_TRAFFIC.get((args.workload, "stream_collide")),
+                    "kernel": "stream_collide"}
+    m["sc"]["traffic"] = PMC_TRAFFIC.get((args.workload, "stream_collide"))
     out = {
-        "metric": METRIC, "value": round(mlups, 3), "unit": "MLUPs/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True,
+        "metric": METRIC, "value": round(m["mlups"], 3), "unit": "MLUPs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(m["dt"] / args.steps * 1e3, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": {0: "fp16s storage / f32", 1: "fp16c storage / f32", 2: "f32"}[int(cfg.float_type)],
         "data": "synthetic", "config": {"workload": args.workload, "lattice": [cfg.n_x, cfg.n_y, cfg.n_z],
                                         "velocity_set": cfg.velocity_set.name, "float_type": cfg.float_type.name,
                                         "mhd": bool(cfg.ext_magneto_hydro), "lod_depth": cfg.mhd_lod_depth,
-                                        "field_mode": args.field_mode, "fluid_cells": n_f, **meta},
-        "roofline": roofline, "kernels": kernels,
+                                        "field_mode": args.field_mode, "fluid_cells": n_f, **m["meta"]},
+        "roofline": roofline, "kernels": kernels, "state": state,
     }
-    _, copy_gbs = I.hbm_copy_probe(1 << 30, 10)
-    out["hbm_copy_probe"] = {"GB/s": round(copy_gbs, 1), "frac_of_peak": round(copy_gbs / HBM_PEAK_GBS, 4),
-                             "note": "best of 10 nontemporal 16-byte copies of 1 GiB (read + write)"}
-    kernels["stream_collide"]["frac_of_copy_probe"] = round(kernels["stream_collide"]["GB/s"] / copy_gbs, 4) \
-        if kernels["stream_collide"]["GB/s"] else None
-    # the same access shape as stream_collide (one work-item per cell, 19 or 45
-    # SoA streams of the storage element) without its arithmetic
-    elem = 4 if int(cfg.float_type) == 2 else 2
-    streams = 45 if cfg.ext_magneto_hydro else 19
-    cells = min(N, 1 << 24)
-    _, pat_gbs = I.hbm_stream_probe(elem, streams, cells, 10)
-    out["pattern_probe"] = {"GB/s": round(pat_gbs, 1), "frac_of_peak": round(pat_gbs / HBM_PEAK_GBS, 4),
-                            "streams": streams, "bytes_per_element": elem, "cells": cells,
-                            "note": "one work-item per cell loading its SoA stream elements then storing them to "
-                                    "a second buffer, nontemporal, best of 10 (read + write)"}
-    kernels["stream_collide"]["frac_of_pattern_probe"] = round(kernels["stream_collide"]["GB/s"] / pat_gbs, 4) \
-        if kernels["stream_collide"]["GB/s"] else None
-    # the same lattice with stream_collide's in-place esoteric-pull addressing
-    # (half the arrays at the neighbour n + c_i) and no arithmetic: the access
-    # pattern's own ceiling
-    if N <= (1 << 27) and cfg.velocity_set.name == "D3Q19":
-        _, pull_gbs = I.hbm_pull_probe(elem, streams, cfg.n_x, cfg.n_y, cfg.n_z, 10)
-        out["pull_probe"] = {"GB/s": round(pull_gbs, 1), "frac_of_peak": round(pull_gbs / HBM_PEAK_GBS, 4),
-                             "streams": streams, "bytes_per_element": elem, "lattice": [cfg.n_x, cfg.n_y, cfg.n_z],
-                             "note": "one work-item per cell reading every SoA array at the cell or at n + c_i "
-                                     "(esoteric pull, even step), then rewriting the same slots, nontemporal, "
-                                     "best of 10 (read + write)"}
-        kernels["stream_collide"]["frac_of_pull_probe"] = round(kernels["stream_collide"]["GB/s"] / pull_gbs, 4) \
-            if kernels["stream_collide"]["GB/s"] else None
+    probes(out, kernels, cfg, N)
     if thruster:
         out["surrogate"] = {k2: v for k2, v in lbm.surrogate_stats().items()}
         out["surrogate"]["note"] = ("field steps of the whole run including warm-up; the first is the direct "
@@ -427,6 +542,12 @@ def main():
     elif cfg.ext_magneto_hydro and not args.no_surrogate and cfg.mhd_lod_depth >= 3:
         out["surrogate"] = surrogate_leg(lbm, cfg, train_steps=args.surrogate_train)
     lbm.close()
+    if cfg.ext_magneto_hydro and not thruster and int(cfg.float_type) == 2 and not args.no_fp16c:
+        parts = args.workload.split("-")
+        rest = [p for p in parts[1:] if p not in ("fp32", "fp16s", "fp16c")]
+        out["fp16c"] = fp16c_leg("-".join([parts[0], "fp16c"] + rest), args.steps, args.warmup, mode)
+    if not thruster and not args.no_fp64_error and cfg.velocity_set.name == "D3Q19" and not cfg.ext_subgrid_ecr:
+        out["fp64_error"] = fp64_error(args.workload, args.warmup + args.steps, mode)
     if cfg.ext_magneto_hydro and not thruster and not args.no_static_field:
         out["static_field_precompute"] = static_field_leg()
     if not args.no_cpu_baseline:

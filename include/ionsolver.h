@@ -29,7 +29,8 @@ typedef enum ion_status {
     ION_ERR_UNSUPPORTED = -2, /* valid for the reference, not for this backend */
     ION_ERR_DEVICE = -3,      /* HIP runtime error */
     ION_ERR_IO = -4,          /* file error */
-    ION_ERR_FORMAT = -5       /* malformed save file */
+    ION_ERR_FORMAT = -5,      /* malformed save file */
+    ION_ERR_NONFINITE = -6    /* the watchdog found a non-finite mass or charge sum */
 } ion_status;
 
 /* VelocitySet, src/lbm/types.rs:16-26 (same discriminants) */
@@ -61,7 +62,12 @@ typedef struct ion_units {
  * the D3Q7 charge DDFs are reset to their equilibrium with that charge every
  * step (no collision), and Q is not written back. It is not part of the save
  * format: ion_load takes it from cfg_inout, like the other fields the file
- * does not store. */
+ * does not store. electron_density0 has no reference counterpart either:
+ * initialize writes the electron-gas DDFs of non-solid cells as the
+ * equilibrium of density electron_density0 (the reference writes density 0,
+ * src/kernels/sim_kernels.cl:823-825, which leaves the electron density at the
+ * rounding residue of the shifted sum); 0 (the default) is the reference. Not
+ * in the save format either. */
 typedef struct ion_config {
     int32_t velocity_set;
     int32_t relaxation_time;
@@ -82,6 +88,7 @@ typedef struct ion_config {
     float ecr_freq;
     float ecr_field_strength;
     uint64_t run_steps;
+    float electron_density0;
 } ion_config;
 
 /* Device buffers of one domain (src/lbm/domain.rs:45-74), all SoA:
@@ -195,6 +202,15 @@ int ion_load(const char* path, ion_config* cfg_inout, ion_handle** out);
 
 /* Float64 conservation sums over fluid cells: sum(rho), sum(Q). */
 int ion_conservation(ion_handle* h, double* mass, double* charge);
+/* NaN/Inf watchdog (no reference counterpart; the reference has no failure
+ * detection): with every > 0, ion_run and ion_do_time_step compute the
+ * float64 mass and charge sums of ion_conservation after every step whose
+ * counter t is a multiple of `every`, and stop with ION_ERR_NONFINITE at the
+ * first non-finite one (the state of that step stays readable). 0 (the
+ * default) switches it off. ion_watchdog_status: the step of the first
+ * non-finite sum (UINT64_MAX if none), and the sums of the last check. */
+int ion_set_watchdog(ion_handle* h, uint32_t every);
+int ion_watchdog_status(const ion_handle* h, uint64_t* first_nonfinite_step, double* mass, double* charge);
 
 int ion_set_field_mode(ion_handle* h, int mode);
 int ion_set_timing(ion_handle* h, int enable);

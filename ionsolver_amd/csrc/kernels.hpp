@@ -45,6 +45,7 @@ struct LbmArgs {
     float kkge, wq;       // DEF_KKGE, DEF_WQ
     float ils;            // 1 / cells per LOD block
     double ils_d;         // the same quotient in double (FP64 instantiation)
+    float ei_rho0;        // initialize: electron-gas density of non-solid cells (build extension; 0 = reference)
 };
 
 // Arguments of the E/B field update.

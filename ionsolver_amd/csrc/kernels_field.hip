@@ -286,21 +286,22 @@ __global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b_f64(FieldArgs a) {
 
 // The field update on LOD-aligned tiles, FAST or EXACT arithmetic. A block covers
 // (ROW*TX) x TY x TZ cells inside one top-level LOD block, so the own-LOD
-// skip is the same for every cell of the block: the thread that stages the
-// own source into LDS records its tile slot, and the source loop runs in two
-// branch-free segments around it. Each thread owns ROW consecutive cells of
-// one row: the source terms that depend only on y and z (r_y, r_z,
-// r_y^2 + r_z^2 and the x component of v x r) are computed once per source
-// and shared by the ROW cells. EXACT keeps the reference's arithmetic per
-// interaction (only r_y^2 and r_z^2 are shared, the sum order is unchanged)
-// with the specialised correctly rounded operators of cr_math.hpp: |r|^2 is a
+// skip is the same for every cell of the block: a wave-uniform branch on the
+// source's LOD index. Each thread owns ROW consecutive cells of one row: the
+// source terms that depend only on y and z (r_y, r_z, r_y^2 + r_z^2 and the
+// x component of v x r) are computed once per source and shared by the ROW
+// cells. The sources are wave-uniform: they are read straight from the
+// compacted list at a uniform index, which the compiler turns into scalar
+// loads (SGPR operands for the VALU, no LDS staging and no barriers in the
+// loop). A wavefront whose cells are all solid skips the sweep (its cells
+// are not written). EXACT keeps the reference's arithmetic per interaction
+// (only r_y^2 and r_z^2 are shared, the sum order is unchanged) with the
+// specialised correctly rounded operators of cr_math.hpp: |r|^2 is a
 // multiple of 1/4 in [1/4, 2^24) (cell minus block centre), |r|^3 lies in
 // [1/8, 2^36] and the dividends r_i are 0 or in [1/2, 2^11].
 template <int ROW, bool FAST>
 __global__ void __launch_bounds__(256) k_update_e_b_tiled(FieldArgs a, uint32_t TX, uint32_t TY, uint32_t TZ) {
-    __shared__ float4 s_pq[FIELD_TILE];
-    __shared__ float4 s_vd[FIELD_TILE];
-    __shared__ uint32_t s_own;
+    __shared__ float s_epi[ROW * 256];
     const uint32_t NX = a.nx, NY = a.ny;
     const uint32_t tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);
     const uint32_t tiles_x = NX / (ROW * TX), tiles_y = NY / TY;
@@ -316,77 +317,90 @@ __global__ void __launch_bounds__(256) k_update_e_b_tiled(FieldArgs a, uint32_t 
         px[k] = (float)(x0 + k);
         ex[k] = ey[k] = ez[k] = bxa[k] = bya[k] = bza[k] = 0.0f;
     }
-    auto source = [&](uint32_t s) {
-        const float4 vd = s_vd[s];
-        const float4 pq = s_pq[s];
-        const float ry = py - pq.y, rz = pz - pq.z;
-        if constexpr (FAST) {
-            const float syz = __builtin_fmaf(ry, ry, rz * rz);
-            const float cx = __builtin_fmaf(vd.y, rz, -(vd.z * ry));  // (v x r).x
-            const float m1 = vd.x * rz, m2 = vd.x * ry;
+    // does any of this work-item's cells need the field?
+    const uint8_t* const fl = a.flags + (uint64_t)x0 + ((uint64_t)y + (uint64_t)z * NY) * NX;
+    bool fluid = false;
 #pragma unroll
-            for (int k = 0; k < ROW; k++) {
-                const float rx = px[k] - pq.x;
-                const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, syz));
-                const float t = pq.w * (inv * inv * inv);
-                ex[k] = __builtin_fmaf(t, rx, ex[k]);
-                ey[k] = __builtin_fmaf(t, ry, ey[k]);
-                ez[k] = __builtin_fmaf(t, rz, ez[k]);
-                bxa[k] = __builtin_fmaf(t, cx, bxa[k]);
-                bya[k] = __builtin_fmaf(t, __builtin_fmaf(vd.z, rx, -m1), bya[k]);  // (v x r).y
-                bza[k] = __builtin_fmaf(t, __builtin_fmaf(-vd.y, rx, m2), bza[k]);  // (v x r).z
-            }
-        } else {
-            // interact_exact with r_y^2, r_z^2 hoisted: pre = r / |r|^3,
-            // e += q pre, b += q (v x pre), every operation rounded on its own
-            const float ry2 = ry * ry, rz2 = rz * rz;
-            const float q = pq.w;
-#pragma unroll
-            for (int k = 0; k < ROW; k++) {
-                const float rx = px[k] - pq.x;
-                const float sm = sqrt_cr((rx * rx + ry2) + rz2);
-                const float cbm = (sm * sm) * sm;
-                const float prx = div_cr(rx, cbm), pry = div_cr(ry, cbm), prz = div_cr(rz, cbm);
-                ex[k] = ex[k] + q * prx;
-                ey[k] = ey[k] + q * pry;
-                ez[k] = ez[k] + q * prz;
-                const float crx = vd.y * prz - vd.z * pry, cry = vd.z * prx - vd.x * prz, crz = vd.x * pry - vd.y * prx;
-                bxa[k] = bxa[k] + q * crx;
-                bya[k] = bya[k] + q * cry;
-                bza[k] = bza[k] + q * crz;
-            }
-        }
-    };
+    for (int k = 0; k < ROW; k++) fluid |= (fl[k] & 0x1F) != 0x01;
+    const float4* __restrict__ const spq = a.src_pq;
+    const float4* __restrict__ const svd = a.src_vd;
     const uint32_t count = *a.src_count;
-    for (uint32_t base = 0; base < count; base += FIELD_TILE) {
-        const uint32_t m = min(FIELD_TILE, count - base);
-        __syncthreads();
-        if (threadIdx.x == 0) s_own = m;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-            const float4 vd = a.src_vd[base + i];
-            s_pq[i] = a.src_pq[base + i];
-            s_vd[i] = vd;
-            if (__float_as_uint(vd.w) == own) s_own = i;  // at most one source is the own LOD
-        }
-        __syncthreads();
-        const uint32_t skip = s_own;
+    // The sources are in ascending LOD index: the own LOD's position (if it
+    // has a charge) is found by a bisection, and the sweep runs in two
+    // branch-free ranges around it (the reference skips it, sim_kernels.cl:950).
+    uint32_t lo = 0, hi = count;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (__float_as_uint(svd[mid].w) < own) lo = mid + 1;
+        else hi = mid;
+    }
+    const uint32_t skip = (lo < count && __float_as_uint(svd[lo].w) == own) ? lo : count;
+    auto sweep = [&](uint32_t s0, uint32_t s1) {
+        if (s0 >= s1) return;
+        // the next source's scalar loads are issued before this one's
+        // arithmetic, so their latency hides behind it
+        float4 vd_next = svd[s0], pq_next = spq[s0];
         // one source per iteration: the compiler then lays the ROW cells'
         // chains side by side (measured at 256^3, FAST: 17.0 ms against
         // 18.1 ms with two sources per iteration and 17.5 ms with four)
 #pragma unroll 1
-        for (uint32_t s = 0; s < skip; s++) source(s);
-#pragma unroll 1
-        for (uint32_t s = skip + 1; s < m; s++) source(s);
+        for (uint32_t s = s0; s < s1; s++) {
+            const float4 vd = vd_next, pq = pq_next;
+            const uint32_t sn = s + 1 < s1 ? s + 1 : s;
+            vd_next = svd[sn];
+            pq_next = spq[sn];
+            const float ry = py - pq.y, rz = pz - pq.z;
+            if constexpr (FAST) {
+                const float syz = __builtin_fmaf(ry, ry, rz * rz);
+                const float cx = __builtin_fmaf(vd.y, rz, -(vd.z * ry));  // (v x r).x
+                const float m1 = vd.x * rz, m2 = vd.x * ry;
+#pragma unroll
+                for (int k = 0; k < ROW; k++) {
+                    const float rx = px[k] - pq.x;
+                    const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, syz));
+                    const float t = pq.w * (inv * inv * inv);
+                    ex[k] = __builtin_fmaf(t, rx, ex[k]);
+                    ey[k] = __builtin_fmaf(t, ry, ey[k]);
+                    ez[k] = __builtin_fmaf(t, rz, ez[k]);
+                    bxa[k] = __builtin_fmaf(t, cx, bxa[k]);
+                    bya[k] = __builtin_fmaf(t, __builtin_fmaf(vd.z, rx, -m1), bya[k]);  // (v x r).y
+                    bza[k] = __builtin_fmaf(t, __builtin_fmaf(-vd.y, rx, m2), bza[k]);  // (v x r).z
+                }
+            } else {
+                // interact_exact with r_y^2, r_z^2 hoisted: pre = r / |r|^3,
+                // e += q pre, b += q (v x pre), every operation rounded on its own
+                const float ry2 = ry * ry, rz2 = rz * rz;
+                const float q = pq.w;
+#pragma unroll
+                for (int k = 0; k < ROW; k++) {
+                    const float rx = px[k] - pq.x;
+                    const float sm = sqrt_cr((rx * rx + ry2) + rz2);
+                    const float cbm = (sm * sm) * sm;
+                    const float prx = div_cr(rx, cbm), pry = div_cr(ry, cbm), prz = div_cr(rz, cbm);
+                    ex[k] = ex[k] + q * prx;
+                    ey[k] = ey[k] + q * pry;
+                    ez[k] = ez[k] + q * prz;
+                    const float crx = vd.y * prz - vd.z * pry, cry = vd.z * prx - vd.x * prz,
+                                crz = vd.x * pry - vd.y * prx;
+                    bxa[k] = bxa[k] + q * crx;
+                    bya[k] = bya[k] + q * cry;
+                    bza[k] = bza[k] + q * crz;
+                }
+            }
+        }
+    };
+    if (__any(fluid)) {
+        sweep(0, skip);
+        sweep(skip + 1, count);
     }
     // Epilogue through LDS, one component at a time: each work-item's ROW
     // consecutive cells go to LDS in block order, then the block writes whole
     // row segments (ROW*TX cells) with consecutive lanes on consecutive cells,
-    // so that every store (and E_stat / B_stat load) is contiguous.
+    // so that every store (and E_stat / B_stat load) is contiguous. Solid
+    // cells (among them every cell of a wavefront that skipped the sweep) are
+    // not written.
     const uint64_t N = a.N;
-    const uint32_t W = ROW * TX, cells = ROW * blockDim.x;
-    static_assert(ROW * 256 <= FIELD_TILE * 4, "the epilogue stages ROW floats per work-item in s_pq");
-    float* const lds = (float*)s_pq;  // cells floats fit in the source tile (at most 256 work-items)
+    const uint32_t W = ROW * TX;
     const uint32_t own0 = x0 - bx * W + W * (ty + TY * tz);  // first own cell in block order
     uint32_t live = 0;  // bit j: cell threadIdx.x + j * blockDim.x is not solid
     uint64_t gn[ROW];
@@ -397,21 +411,20 @@ __global__ void __launch_bounds__(256) k_update_e_b_tiled(FieldArgs a, uint32_t 
         gn[j] = (uint64_t)(bx * W + lx) + ((uint64_t)(by * TY + ly) + (uint64_t)(bz * TZ + lz) * NY) * NX;
         if ((a.flags[gn[j]] & 0x1F) != 0x01) live |= 1u << j;
     }
-    (void)cells;
 #pragma unroll
     for (int c = 0; c < 6; c++) {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < ROW; k++) {
             const float v = c == 0 ? ex[k] : c == 1 ? ey[k] : c == 2 ? ez[k] : c == 3 ? bxa[k] : c == 4 ? bya[k] : bza[k];
-            lds[own0 + k] = v;
+            s_epi[own0 + k] = v;
         }
         __syncthreads();
         const uint64_t off = (uint64_t)(c % 3) * N;
 #pragma unroll
         for (int j = 0; j < ROW; j++) {
             if (!(live >> j & 1u)) continue;
-            const float v = lds[threadIdx.x + j * blockDim.x];
+            const float v = s_epi[threadIdx.x + j * blockDim.x];
             const uint64_t n = gn[j];
             if (a.raw != nullptr) {
                 a.raw[(uint64_t)c * N + n] = v;

@@ -172,7 +172,9 @@ __global__ void __launch_bounds__(256) k_initialize(LbmArgs a) {
             Bd[k * N + n] = Bs[k * N + n];
             Ed[k * N + n] = Es[k * N + n];
         }
-        f_eq<Q>(R(0), ux, uy, uz, fe);
+        // sim_kernels.cl:823-825 writes density 0; a non-zero ei_rho0 (build
+        // extension) starts the electron gas of non-solid cells at that density
+        f_eq<Q>(bo == 0x01 ? R(0) : R(a.ei_rho0), ux, uy, uz, fe);
         store_ddfs<Q, Q, S>((typename S::T*)a.ei, N, n, b, 1u, fe);
         if constexpr (ECR) {  // electron temperature DDFs, sim_kernels.cl:827-831
             a_eq(R(a.et[n]), ux, uy, uz, qeq);

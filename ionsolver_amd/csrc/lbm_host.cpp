@@ -89,6 +89,10 @@ struct ion_handle {
     bool initialized = false;
     float fx = 0, fy = 0, fz = 0;
     int field_mode = ION_FIELD_EXACT;
+    // NaN/Inf watchdog
+    uint32_t watch_every = 0;
+    uint64_t watch_first_bad = UINT64_MAX;
+    double watch_mass = 0.0, watch_charge = 0.0;
     // timing
     bool timing = false;
     ion_timing tm{};
@@ -216,6 +220,7 @@ ion::LbmArgs lbm_args(ion_handle* h) {
     a.ffield = h->cfg.ext_force_field ? 1u : 0u;
     a.upd = h->cfg.graphics_active ? 1u : 0u;
     a.qin = h->cfg.ext_charge_inlet ? 1u : 0u;
+    a.ei_rho0 = h->cfg.electron_density0;
     a.eti = h->eti;
     a.et = h->et;
     a.e_var = h->e_var;
@@ -378,6 +383,10 @@ int validate(const ion_config& c) {
         if (c.velocity_set != ION_D3Q19 || c.float_type == ION_FP64)
             return fail(ION_ERR_UNSUPPORTED, "ext_subgrid_ecr is built for D3Q19 with FP32, FP16S or FP16C storage");
     }
+    if (!(c.electron_density0 >= 0.0f) || !std::isfinite(c.electron_density0))
+        return fail(ION_ERR_INVALID, "electron_density0 must be finite and >= 0");
+    if (c.electron_density0 != 0.0f && !c.ext_magneto_hydro)
+        return fail(ION_ERR_INVALID, "electron_density0 needs ext_magneto_hydro");
     if (c.ext_force_field && !c.ext_volume_force) return fail(ION_ERR_INVALID, "ext_force_field needs ext_volume_force");
     if (c.ext_charge_inlet && !(c.ext_magneto_hydro && c.ext_equilibrium_boudaries))
         return fail(ION_ERR_INVALID, "ext_charge_inlet needs ext_magneto_hydro and ext_equilibrium_boudaries");
@@ -582,21 +591,51 @@ int ion_initialize(ion_handle* h) {
     return ION_OK;
 }
 
+// The watchdog's check after a step: float64 mass and charge sums, at the
+// steps whose counter is a multiple of watch_every.
+static int watchdog(ion_handle* h) {
+    if (!h->watch_every || h->t % h->watch_every) return ION_OK;
+    int rc = ion_conservation(h, &h->watch_mass, &h->watch_charge);
+    if (rc) return rc;
+    if (std::isfinite(h->watch_mass) && std::isfinite(h->watch_charge)) return ION_OK;
+    if (h->watch_first_bad == UINT64_MAX) h->watch_first_bad = h->t;
+    return fail(ION_ERR_NONFINITE, "watchdog: non-finite sum at step " + std::to_string(h->t) + " (mass " +
+                                       std::to_string(h->watch_mass) + ", charge " +
+                                       std::to_string(h->watch_charge) + ")");
+}
+
 int ion_do_time_step(ion_handle* h) {
     if (!h) return fail(ION_ERR_INVALID, "null handle");
     int rc = do_step(h);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
-    return ION_OK;
+    return watchdog(h);
 }
 
 int ion_run(ion_handle* h, uint64_t steps) {
     if (!h) return fail(ION_ERR_INVALID, "null handle");
     int rc;
     if (!h->initialized && (rc = ion_initialize(h))) return rc;
-    for (uint64_t i = 0; i < steps; i++)
+    for (uint64_t i = 0; i < steps; i++) {
         if ((rc = do_step(h))) return rc;
+        if ((rc = watchdog(h))) return rc;
+    }
     HIPCHK(hipStreamSynchronize(h->stream));
+    return ION_OK;
+}
+
+int ion_set_watchdog(ion_handle* h, uint32_t every) {
+    if (!h) return fail(ION_ERR_INVALID, "null handle");
+    h->watch_every = every;
+    h->watch_first_bad = UINT64_MAX;
+    return ION_OK;
+}
+
+int ion_watchdog_status(const ion_handle* h, uint64_t* first_nonfinite_step, double* mass, double* charge) {
+    if (!h) return fail(ION_ERR_INVALID, "null handle");
+    if (first_nonfinite_step) *first_nonfinite_step = h->watch_first_bad;
+    if (mass) *mass = h->watch_mass;
+    if (charge) *charge = h->watch_charge;
     return ION_OK;
 }
 

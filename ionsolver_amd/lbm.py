@@ -103,7 +103,7 @@ class _Config(ctypes.Structure):
         ("ext_subgrid_ecr", ctypes.c_uint8), ("mhd_lod_depth", ctypes.c_uint8),
         ("graphics_active", ctypes.c_uint8), ("ext_charge_inlet", ctypes.c_uint8),
         ("ecr_freq", ctypes.c_float), ("ecr_field_strength", ctypes.c_float),
-        ("run_steps", ctypes.c_uint64),
+        ("run_steps", ctypes.c_uint64), ("electron_density0", ctypes.c_float),
     ]
 
 
@@ -170,6 +170,8 @@ ABI = {
     "ion_set_timing": (_I, [_VP, _I]),
     "ion_get_timing": (_I, [_VP, ctypes.POINTER(_Timing)]),
     "ion_device_codec": (_I, [_I, _I, _VP, _VP, _U64]),
+    "ion_set_watchdog": (_I, [_VP, ctypes.c_uint32]),
+    "ion_watchdog_status": (_I, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "ion_last_error": (ctypes.c_char_p, []),
     "ion_surrogate_config_default": (None, [ctypes.POINTER(_SurConfig)]),
     "ion_surrogate_create": (_I, [_VP, ctypes.POINTER(_SurConfig)]),
@@ -291,6 +293,9 @@ class LbmConfig:
         self.ecr_field_strength = 0.0
         self.graphics_config = GraphicsConfig()
         self.run_steps = 0
+        # no reference counterpart: initial electron-gas density of non-solid
+        # cells (0 = the reference's initialize)
+        self.electron_density0 = 0.0
 
     new = classmethod(lambda cls: cls())
 
@@ -315,6 +320,7 @@ class LbmConfig:
         c.ecr_freq = self.ecr_freq
         c.ecr_field_strength = self.ecr_field_strength
         c.run_steps = self.run_steps
+        c.electron_density0 = self.electron_density0
         return c
 
     @classmethod
@@ -338,6 +344,7 @@ class LbmConfig:
         cfg.mhd_lod_depth = c.mhd_lod_depth
         cfg.graphics_config.graphics_active = bool(c.graphics_active)
         cfg.ecr_freq, cfg.ecr_field_strength, cfg.run_steps = c.ecr_freq, c.ecr_field_strength, c.run_steps
+        cfg.electron_density0 = c.electron_density0
         return cfg
 
     def kernel_constants(self):
@@ -441,6 +448,17 @@ class Lbm:
         q = ctypes.c_double()
         _check(self._lib.ion_conservation(self._h, ctypes.byref(m), ctypes.byref(q)))
         return m.value, q.value
+
+    def set_watchdog(self, every):
+        """Check the float64 mass and charge sums every `every` steps; a
+        non-finite one stops run/do_time_step with IonError (0: off)."""
+        _check(self._lib.ion_set_watchdog(self._h, every))
+
+    def watchdog_status(self):
+        """(first step with a non-finite sum or None, mass, charge of the last check)"""
+        t, m, q = _U64(), _D(), _D()
+        _check(self._lib.ion_watchdog_status(self._h, ctypes.byref(t), ctypes.byref(m), ctypes.byref(q)))
+        return (None if t.value == 2 ** 64 - 1 else t.value), m.value, q.value
 
     def set_field_mode(self, mode):
         _check(self._lib.ion_set_field_mode(self._h, int(mode)))

@@ -17,6 +17,23 @@ from .lbm import FloatType, LbmConfig, ModelType, VelocitySet
 MHD_UNITS = dict(lbm_length=128.0, lbm_velocity=1.0, lbm_rho=1.0, lbm_charge=1.0, lbm_temp=1.0, si_length=0.1,
                  si_velocity=1.0, si_rho=1.2250, si_charge=7e-13, si_temp=1.0)
 
+# Units of the plasma workloads (Units fields set directly): KE = 1.0e-3,
+# KMU = 2.8e-4, KKGE = -2.50 in lattice units. With MHD_UNITS KKGE is
+# -6.8e-15, so the electron-gas velocity shift F / (2 rho_e KKGE)
+# (sim_kernels.cl:651-653) saturates at the clamp +-c for any field and the
+# electron gas runs away within a few steps whatever its density; here it
+# moves by (E + u_e x B) / 5 per step and stays in its stable range.
+PLASMA_UNITS = dict(m=1.58e8, kg=2.27e-12, s=1.0, a=1.0)
+
+
+def use_plasma_units(cfg, n0):
+    """PLASMA_UNITS and an initial electron gas of density n0 on non-solid
+    cells (LbmConfig.electron_density0, a build extension: the reference's
+    initialize starts it at density 0)."""
+    for k, v in PLASMA_UNITS.items():
+        setattr(cfg.units, k, v)
+    cfg.electron_density0 = n0
+
 
 def cell_coords(nx, ny, nz):
     z, y, x = np.meshgrid(np.arange(nz, dtype=np.float64), np.arange(ny, dtype=np.float64),
@@ -47,13 +64,18 @@ def fluid_tgv(n=64, tau=0.6, u0=0.05, float_type=FloatType.FP32, velocity_set=Ve
 
 
 def mhd_blobs(n=256, seed=1234, float_type=FloatType.FP32, lod_depth=4, n_blobs=8, sigma=6.0, amp=0.05,
-              b0=0.05, dipole=3200.0, cyl_r=40.0, nu=None, shape=None):
+              b0=0.05, dipole=3200.0, cyl_r=40.0, nu=None, shape=None, plasma=None):
     """D3Q19 BGK + MHD: Gaussian charge blobs, uniform B0 plus a point dipole,
     Guo Lorentz forcing, a solid z-aligned cylinder (flags 0x01).
 
     Blob centres are uniform in [2 sigma, n - 2 sigma)^3 outside the
     cylinder; sign +-1 and amplitude U[0.5, 1]*amp come from the same seeded
-    stream. Returns (cfg, arrays, blobs)."""
+    stream. With plasma = n0 the blobs sit on a neutral background: the ion
+    charge density written to Q (the charge DDFs' density) is n0 + blobs and
+    the electron gas starts at density n0 (use_plasma_units), so that the cell
+    charge Q = rho_q - rho_e is the blobs; otherwise (the reference's initial
+    state) the charge density is the blobs and the electron gas starts at 0.
+    Returns (cfg, arrays, blobs)."""
     nx, ny, nz = shape if shape is not None else (n, n, n)
     cfg = LbmConfig()
     cfg.velocity_set = VelocitySet.D3Q19
@@ -73,7 +95,11 @@ def mhd_blobs(n=256, seed=1234, float_type=FloatType.FP32, lod_depth=4, n_blobs=
     flags = np.where(solid, 1, 0).astype(np.uint8)
     blobs = []
     lo = 2 * sigma
+    tries = 0
     while len(blobs) < n_blobs:
+        tries += 1
+        if tries > 100000:
+            raise ValueError(f"no room for {n_blobs} blobs of sigma {sigma} beside the cylinder on {nx}x{ny}x{nz}")
         c = rng.uniform([lo, lo, lo], [nx - lo, ny - lo, nz - lo])
         if np.hypot(c[0] - cx, c[1] - cy) < cyl_r * scale + 2 * sigma:
             continue
@@ -83,6 +109,9 @@ def mhd_blobs(n=256, seed=1234, float_type=FloatType.FP32, lod_depth=4, n_blobs=
     q = np.zeros(nx * ny * nz)
     for bx, by, bz, a in blobs:
         q += a * np.exp(-((x - bx) ** 2 + (y - by) ** 2 + (z - bz) ** 2) / (2 * sigma * sigma))
+    if plasma is not None:
+        use_plasma_units(cfg, plasma)
+        q += plasma
     q[solid] = 0.0
     # B_stat = B0 z + point dipole m z at the domain centre: (3 (m.r^) r^ - m) / |r|^3
     rx, ry, rz = x - cx, y - cy, z - 0.5 * nz
@@ -169,7 +198,7 @@ def _loop_field(r, z, a, wire):
 
 
 def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magnet_r=350.0, b_center=0.05,
-             u_inlet=0.02, q_inlet=0.02, n_blobs=8, sigma=12.0, amp=0.05, seed=1234, log=None):
+             u_inlet=0.02, q_inlet=0.02, n_blobs=8, sigma=12.0, amp=0.05, seed=1234, log=None, plasma=None):
     """Synthetic plasma-thruster workload: a z-aligned quartz tube (solid
     shell, flags 0x01) of radius tube_r, a ring magnet (current loop of
     radius magnet_r at mid-height, analytic field scaled to b_center at its
@@ -178,8 +207,11 @@ def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magne
     fluid moving at u_inlet and seeded with Gaussian charge blobs. With
     q_inlet set (None switches it off) the inlet also injects charge: the
     charge inlet extension holds the inlet cells at charge density q_inlet.
-    Lengths are given for n = 1024 and scaled with n. Arrays are built one
-    z-layer at a time. Returns (cfg, arrays, meta)."""
+    With plasma = n0 the charges sit on a neutral background (as in
+    mhd_blobs): PLASMA_UNITS, the electron gas of every non-solid cell starts
+    at density n0 and the ion charge density written to Q is n0 plus the
+    above. Lengths are given for n = 1024 and scaled with n. Arrays are built
+    one z-layer at a time. Returns (cfg, arrays, meta)."""
     from .lbm import FloatType as FT
     ft = FT.FP16C if float_type is None else float_type
     sc = n / 1024.0
@@ -195,10 +227,16 @@ def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magne
     cfg.ext_equilibrium_boudaries = True
     cfg.ext_charge_inlet = q_inlet is not None
     cfg.mhd_lod_depth = lod_depth
+    if plasma is not None:
+        use_plasma_units(cfg, plasma)
+    n0 = 0.0 if plasma is None else plasma
     N = n ** 3
     c = 0.5 * n
     yy, xx = np.meshgrid(np.arange(n, dtype=np.float64) - c, np.arange(n, dtype=np.float64) - c, indexing="ij")
     r = np.sqrt(xx * xx + yy * yy)
+    # the loop field depends on r only: evaluated once per distinct radius of a layer
+    r_u, r_inv = np.unique(r, return_inverse=True)
+    r_inv = r_inv.reshape(r.shape)
     inside = r < rt
     shell = (r >= rt) & (r < rt + th)
     flags = np.zeros(N, np.uint8)
@@ -215,23 +253,27 @@ def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magne
             continue
         blobs.append((p[0], p[1], p[2], (1.0 if rng.uniform() < 0.5 else -1.0) * amp * rng.uniform(0.5, 1.0)))
     s2 = 2.0 * (sigma * sc) ** 2
-    gxy = [a * np.exp(-((xx + c - bx) ** 2 + (yy + c - by) ** 2) / s2) for bx, by, bz, a in blobs]
+    gxy = np.stack([a * np.exp(-((xx + c - bx) ** 2 + (yy + c - by) ** 2) / s2) for bx, by, bz, a in blobs])
+    gxy = (gxy * inside).reshape(len(blobs), -1)
     lay = n * n
+    # per-layer templates: flags (the shell solid; the inlet plane at z = 0),
+    # u_z inside the tube, the background charge density off the shell
+    f_layer = np.where(shell, 1, 0).astype(np.uint8).ravel()
+    f_inlet = np.where(inside, 2, f_layer.reshape(n, n)).astype(np.uint8).ravel()
+    uz_layer = np.where(inside, u_inlet, 0.0).astype(np.float32).ravel()
+    q_back = np.where(shell, 0.0, n0).ravel()
+    q_in = (np.where(inside, q_inlet, 0.0).ravel() + q_back) if q_inlet is not None else None
     for z in range(n):
         sl = slice(z * lay, (z + 1) * lay)
-        f = np.zeros(lay, np.uint8)
-        f[shell.ravel()] = 1
-        if z == 0:
-            f[inside.ravel()] = 2
-        flags[sl] = f
-        u[2 * N + z * lay:2 * N + (z + 1) * lay] = np.where(inside, u_inlet, 0.0).ravel()
-        qz = np.zeros((n, n))
-        for g, (bx, by, bz, a) in zip(gxy, blobs):
-            qz += g * np.exp(-((z - bz) ** 2) / s2)
-        if z == 0 and q_inlet is not None:
-            qz = np.full_like(qz, q_inlet)
-        q[sl] = np.where(inside, qz, 0.0).ravel()
-        brl, bzl = _loop_field(r, np.full_like(r, z - c), rm, 2.0)
+        flags[sl] = f_inlet if z == 0 else f_layer
+        u[2 * N + z * lay:2 * N + (z + 1) * lay] = uz_layer
+        if z == 0 and q_in is not None:
+            q[sl] = q_in
+        else:
+            wz = np.array([np.exp(-((z - bz) ** 2) / s2) for bx, by, bz, a in blobs])
+            q[sl] = wz @ gxy + q_back
+        brl, bzl = _loop_field(r_u, np.full_like(r_u, z - c), rm, 2.0)
+        brl, bzl = brl[r_inv], bzl[r_inv]
         with np.errstate(divide="ignore", invalid="ignore"):
             bxl = np.where(r > 0, brl * xx / r, 0.0)
             byl = np.where(r > 0, brl * yy / r, 0.0)
@@ -242,7 +284,7 @@ def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magne
             log(f"thruster setup: layer {z + 1}/{n}")
     arrays = dict(flags=flags, u=u, q=q, b_stat=b)
     meta = dict(tube_r=rt, wall=th, magnet_r=rm, b_center=b_center, u_inlet=u_inlet, q_inlet=q_inlet,
-                blobs=len(blobs), seed=seed)
+                blobs=len(blobs), seed=seed, plasma=plasma)
     return cfg, arrays, meta
 
 

@@ -492,7 +492,8 @@ struct Sim : SimBase {
                     b_dyn[k * N + n] = b_stat[k * N + n];
                     e_dyn[k * N + n] = e_stat[k * N + n];
                 }
-                f_eq<Q, R>(R(0), u[n], u[N + n], u[2 * N + n], feq);
+                /* density 0 (:823-825), or the build extension's ei_rho0 off solid cells */
+                f_eq<Q, R>(bo == 0x01 ? R(0) : R(p.ei_rho0), u[n], u[N + n], u[2 * N + n], feq);
                 store<Q>(n, feq, ei.data(), j, tt);
                 if (ecr()) { /* electron temperature DDFs (:827-831) */
                     a_eq<R>(et[n], u[n], u[N + n], u[2 * N + n], qeq);

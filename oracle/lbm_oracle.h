@@ -74,6 +74,8 @@ typedef struct orc_params {
     float fx, fy, fz;         /* volume force kernel arguments */
     float ke, kmu, kkge, wq;  /* DEF_KE, DEF_KMU, DEF_KKGE, DEF_WQ */
     float kkbme, keabs;       /* DEF_KKBME, DEF_KEABS (SUBGRID_ECR) */
+    float ei_rho0;            /* initial electron-gas density of non-solid cells
+                                 (build extension; 0 = sim_kernels.cl:823-825) */
 } orc_params;
 
 /* Buffer ids, one per reference device buffer (src/lbm/domain.rs:151-211). */

@@ -48,7 +48,8 @@ def oracle_for(cfg, float_type=None):
     return O.OracleSim(cfg.n_x, cfg.n_y, cfg.n_z, velocity_set=int(cfg.velocity_set),
                        relaxation_time=int(cfg.relaxation_time), float_type=ft, lod_depth=cfg.mhd_lod_depth,
                        ext=ext, w=k["w"], fx=cfg.f_x, fy=cfg.f_y, fz=cfg.f_z, ke=k["ke"], kmu=k["kmu"],
-                       kkge=k["kkge"], wq=k["wq"], kkbme=k["kkbme"], keabs=k["keabs"])
+                       kkge=k["kkge"], wq=k["wq"], kkbme=k["kkbme"], keabs=k["keabs"],
+                       ei_rho0=cfg.electron_density0)
 
 
 def random_inputs(cfg, rng, solid_frac=0.1, e_frac=0.05, charge=0.05, field=0.05):
@@ -88,7 +89,7 @@ def write_both(gpu, orc, arrays):
 
 def make_config(nx=16, ny=16, nz=16, velocity_set=2, float_type=2, relaxation_time=0, mhd=False, lod_depth=2,
                 eqb=True, vforce=True, ffield=False, graphics=True, force=(1e-5, -2e-5, 3e-5), nu=0.02, units=None,
-                charge_inlet=False, ecr=False):
+                charge_inlet=False, ecr=False, electron_density0=0.0):
     cfg = I.LbmConfig()
     cfg.velocity_set = I.VelocitySet(velocity_set)
     cfg.float_type = I.FloatType(float_type)
@@ -103,6 +104,7 @@ def make_config(nx=16, ny=16, nz=16, velocity_set=2, float_type=2, relaxation_ti
     cfg.ext_charge_inlet = charge_inlet
     cfg.ext_subgrid_ecr = ecr
     cfg.graphics_config.graphics_active = graphics
+    cfg.electron_density0 = electron_density0
     if vforce or mhd:
         cfg.f_x, cfg.f_y, cfg.f_z = force
     if mhd:

@@ -36,7 +36,7 @@ class OrcParams(ctypes.Structure):
         ("lod_depth", ctypes.c_uint32), ("ext", ctypes.c_uint32),
         ("w", ctypes.c_float), ("fx", ctypes.c_float), ("fy", ctypes.c_float), ("fz", ctypes.c_float),
         ("ke", ctypes.c_float), ("kmu", ctypes.c_float), ("kkge", ctypes.c_float), ("wq", ctypes.c_float),
-        ("kkbme", ctypes.c_float), ("keabs", ctypes.c_float),
+        ("kkbme", ctypes.c_float), ("keabs", ctypes.c_float), ("ei_rho0", ctypes.c_float),
     ]
 
 
@@ -165,9 +165,10 @@ class OracleSim:
     """One oracle simulation (mirrors the reference Lbm of one domain)."""
 
     def __init__(self, nx, ny, nz, velocity_set=D3Q19, relaxation_time=0, float_type=FP32, lod_depth=4,
-                 ext=0, w=1.0, fx=0.0, fy=0.0, fz=0.0, ke=0.0, kmu=0.0, kkge=0.0, wq=0.4, kkbme=0.0, keabs=0.0):
+                 ext=0, w=1.0, fx=0.0, fy=0.0, fz=0.0, ke=0.0, kmu=0.0, kkge=0.0, wq=0.4, kkbme=0.0, keabs=0.0,
+                 ei_rho0=0.0):
         p = OrcParams(velocity_set, relaxation_time, float_type, nx, ny, nz, lod_depth, ext,
-                      w, fx, fy, fz, ke, kmu, kkge, wq, kkbme, keabs)
+                      w, fx, fy, fz, ke, kmu, kkge, wq, kkbme, keabs, ei_rho0)
         err = ctypes.create_string_buffer(256)
         h = lib().orc_create(ctypes.byref(p), err)
         if not h:

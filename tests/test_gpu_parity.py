@@ -43,6 +43,17 @@ def test_mhd_d3q19(float_type, lod_depth):
     assert compare_all(gpu, orc, info) == []
 
 
+@pytest.mark.parametrize("float_type,trt", [(2, 0), (1, 1), (0, 0)])
+def test_mhd_electron_density0(float_type, trt):
+    """The electron gas started at a non-zero density (build extension
+    electron_density0), every buffer exact."""
+    rng = np.random.default_rng(350 + float_type * 10 + trt)
+    gpu, orc, info = make_pair(rng, nx=16, ny=16, nz=16, mhd=True, lod_depth=2, float_type=float_type,
+                               relaxation_time=trt, electron_density0=0.75)
+    run_both(gpu, orc, 4)
+    assert compare_all(gpu, orc, info) == []
+
+
 @pytest.mark.parametrize("nx,ny,nz,D,ft", [(512, 4, 4, 2, 1), (384, 8, 8, 2, 2), (1024, 2, 4, 1, 0)])
 def test_mhd_rows_wider_than_a_block(nx, ny, nz, D, ft):
     """Rows longer than 256 cells: stream_collide blocks hold part of a row

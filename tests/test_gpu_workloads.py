@@ -1,0 +1,220 @@
+"""The benchmark's configurations as GPU tests (through the C ABI):
+
+* 64^3 Taylor-Green vortex, D3Q19 BGK, FP32, tau = 0.6, u0 = 0.05, 1000
+  steps: every buffer bit-identical to the oracle;
+* the MHD blob workload (8 Gaussian charge blobs, B0 plus a dipole, the solid
+  cylinder) at 64^3 with LOD depth 3: every buffer bit-identical to the
+  oracle over 20 steps on a neutral plasma background (a live electron gas),
+  and over 10 steps from the reference's own initial state, in which the
+  electron gas starts at the rounding residue of its shifted equilibrium and
+  the NaN it produces spreads (NaN positions must match too);
+* storage formats on the 128^3 plasma blob workload (LOD depth 4, EXACT
+  field arithmetic, 50 steps): the relative L2 error of rho - 1, u, E - E_stat,
+  B - B_stat and Q of FP32, FP16S and FP16C storage against the FP64
+  instantiation of the same kernels, under the bounds stated below;
+* the blob workload at 384^3 (LOD depth 4): FAST field arithmetic within
+  1e-5 of EXACT on identical LOD sums, and gas mass and charge conserved over
+  20 plasma steps in each storage format;
+* the 1024^3 FP16C thruster (quartz tube, ring magnet, charge inlet, 8 blobs,
+  plasma background) with the learned field step judged by the direct kernel:
+  a few steps, finite state, gas mass within the FP16C rounding drift, the
+  judge's residual below 1 %."""
+import numpy as np
+import pytest
+
+import ionsolver_amd as I
+import oracle_lib as O
+from harness import FIELD_OF, compare_all, oracle_for, rel_l2
+from ionsolver_amd import Field, FieldMode, FloatType, scenarios
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_and_oracle(cfg, arrays):
+    gpu = I.Lbm(cfg)
+    orc = oracle_for(gpu.config)
+    for k, v in arrays.items():
+        gpu.write_field(FIELD_OF[k], v)
+        dst = orc.buf(k)
+        dst[:] = v.astype(dst.dtype)
+    return gpu, orc, dict(cfg=gpu.config)
+
+
+def test_tgv64_1000_steps_bit_exact():
+    cfg, arrays = scenarios.fluid_tgv(n=64)
+    gpu, orc, info = gpu_and_oracle(cfg, arrays)
+    try:
+        gpu.initialize()
+        orc.initialize()
+        gpu.run(1000)
+        orc.run(1000)
+        assert gpu.get_time_step() == 1000
+        assert compare_all(gpu, orc, info) == []
+        assert np.abs(gpu.read_field(Field.U)).max() > 0.01  # the vortex is still there
+    finally:
+        gpu.close()
+
+
+@pytest.mark.parametrize("plasma,steps", [(1.0, 20), (None, 10)])
+def test_mhd_blobs64_bit_exact(plasma, steps):
+    cfg, arrays, _ = scenarios.mhd_blobs(n=64, lod_depth=3, plasma=plasma)
+    gpu, orc, info = gpu_and_oracle(cfg, arrays)
+    try:
+        gpu.initialize()
+        orc.initialize()
+        for s in range(steps):
+            gpu.do_time_step()
+            orc.do_time_step()
+            if s in (0, steps // 2):
+                assert compare_all(gpu, orc, info) == [], f"step {s + 1}"
+        assert compare_all(gpu, orc, info) == []
+        finite = np.isfinite(gpu.read_field(Field.U)).all()
+        assert finite if plasma else not finite  # the reference's start turns NaN within 10 steps
+    finally:
+        gpu.close()
+
+
+def test_watchdog_stops_at_the_first_nonfinite_sum():
+    """The reference's initial state turns NaN within 10 steps: the watchdog
+    (float64 mass and charge sums after every step) stops the run there; on
+    the plasma background it runs through."""
+    for plasma in (None, 1.0):
+        cfg, arrays, _ = scenarios.mhd_blobs(n=64, lod_depth=3, plasma=plasma)
+        lbm = I.Lbm(cfg)
+        try:
+            scenarios.apply(lbm, arrays)
+            lbm.initialize()
+            lbm.set_watchdog(1)
+            if plasma is None:
+                with pytest.raises(I.IonError, match="watchdog"):
+                    lbm.run(30)
+                t, m, q = lbm.watchdog_status()
+                assert t is not None and 1 <= t <= 10 and t == lbm.get_time_step()
+                assert not (np.isfinite(m) and np.isfinite(q))
+            else:
+                lbm.run(30)
+                t, m, q = lbm.watchdog_status()
+                assert t is None and np.isfinite(m) and np.isfinite(q) and lbm.get_time_step() == 30
+        finally:
+            lbm.close()
+
+
+# rel-L2 bounds against the FP64 instantiation after 50 steps at 128^3
+FORMAT_BOUNDS = {
+    FloatType.FP32: dict(rho=1e-4, u=1e-4, E=1e-4, B=1e-4, Q=1e-4),
+    FloatType.FP16S: dict(rho=2e-2, u=2e-2, E=5e-2, B=5e-2, Q=5e-2),
+    FloatType.FP16C: dict(rho=2e-2, u=2e-2, E=5e-2, B=5e-2, Q=5e-2),
+}
+
+
+def format_run(ft, n=128, steps=50):
+    cfg, arrays, _ = scenarios.mhd_blobs(n=n, lod_depth=4, float_type=ft, plasma=1.0)
+    lbm = I.Lbm(cfg)
+    try:
+        scenarios.apply(lbm, arrays)
+        lbm.set_field_mode(FieldMode.EXACT)
+        lbm.initialize()
+        lbm.run(steps)
+        r = {k: lbm.read_field(f).astype(np.float64) for k, f in
+             (("rho", Field.RHO), ("u", Field.U), ("E", Field.E_DYN), ("B", Field.B_DYN), ("Q", Field.Q),
+              ("Es", Field.E_STAT), ("Bs", Field.B_STAT))}
+    finally:
+        lbm.close()
+    r["rho"] -= 1.0
+    r["E"] -= r.pop("Es")
+    r["B"] -= r.pop("Bs")
+    return r
+
+
+def test_storage_formats_against_fp64_mhd128():
+    ref = format_run(FloatType.FP64)
+    for k, v in ref.items():
+        assert np.isfinite(v).all() and np.abs(v).max() > 0, k
+    errs = {}
+    for ft, bounds in FORMAT_BOUNDS.items():
+        out = format_run(ft)
+        errs[ft.name] = {k: rel_l2(out[k], ref[k]) for k in bounds}
+    print("rel-L2 vs FP64 after 50 steps:", errs)
+    for ft, bounds in FORMAT_BOUNDS.items():
+        for k, b in bounds.items():
+            assert errs[ft.name][k] < b, (ft.name, k, errs[ft.name][k])
+
+
+def test_mhd384_fast_matches_exact():
+    out = {}
+    for mode in (FieldMode.EXACT, FieldMode.FAST):
+        cfg, arrays, _ = scenarios.mhd_blobs(n=384, lod_depth=4, plasma=1.0)
+        lbm = I.Lbm(cfg)
+        try:
+            scenarios.apply(lbm, arrays)
+            lbm.set_field_mode(mode)
+            lbm.initialize()
+            lbm.do_time_step()
+            out[mode] = [lbm.read_field(f) for f in (Field.QU_LOD, Field.E_DYN, Field.B_DYN, Field.E_STAT,
+                                                       Field.B_STAT)]
+        finally:
+            lbm.close()
+    ex, fa = out[FieldMode.EXACT], out[FieldMode.FAST]
+    assert np.array_equal(ex[0], fa[0])
+    assert np.abs(ex[1] - ex[3]).max() > 0 and np.abs(ex[2] - ex[4]).max() > 0
+    assert rel_l2(fa[1] - fa[3], ex[1] - ex[3]) < 1e-5
+    assert rel_l2(fa[2] - fa[4], ex[2] - ex[4]) < 1e-5
+
+
+def charge_sum(lbm, ft):
+    """Total ion charge density: the charge DDFs over every slot plus N
+    (the stored DDFs are shifted by their weights, which sum to 1 per cell)."""
+    d = lbm.read_field(Field.FQI)
+    if ft == FloatType.FP16C:
+        d = O.fp16c_decode(d)
+    elif ft == FloatType.FP16S:
+        d = O.fp16s_load(d)
+    return float(np.sum(d.astype(np.float64))) + d.size // 7
+
+
+@pytest.mark.parametrize("ft,tol_q", [(FloatType.FP32, 1e-6), (FloatType.FP16S, 1e-3), (FloatType.FP16C, 1e-2)])
+def test_mhd384_conserves_mass_and_charge(ft, tol_q):
+    cfg, arrays, _ = scenarios.mhd_blobs(n=384, lod_depth=4, float_type=ft, plasma=1.0)
+    lbm = I.Lbm(cfg)
+    try:
+        scenarios.apply(lbm, arrays)
+        lbm.set_field_mode(FieldMode.FAST)
+        lbm.initialize()
+        m0, q0 = lbm.conservation()[0], charge_sum(lbm, ft)
+        lbm.run(20)
+        m1, q1 = lbm.conservation()[0], charge_sum(lbm, ft)
+        print(f"{ft.name}: mass {m0} -> {m1}, ion charge {q0} -> {q1}")
+        assert np.isfinite(m1) and abs(m1 - m0) < 1e-6 * m0
+        assert abs(q1 - q0) < tol_q * q0
+        assert np.isfinite(lbm.read_field(Field.U)).all() and np.isfinite(lbm.read_field(Field.Q)).all()
+    finally:
+        lbm.close()
+
+
+def test_thruster1024_fp16c_surrogate_with_judge():
+    cfg, arrays, meta = scenarios.thruster(n=1024, plasma=1.0)
+    lbm = I.Lbm(cfg)
+    try:
+        scenarios.apply(lbm, arrays)
+        flags = arrays["flags"]
+        del arrays
+        lbm.set_field_mode(FieldMode.FAST)
+        lbm.surrogate_create(channels=16)
+        lbm.set_field_solver(I.FieldSolver.SURROGATE, judge_every=4, tolerance=0.01)
+        lbm.initialize()
+        m0 = lbm.conservation()[0]
+        lbm.run(8)
+        m1 = lbm.conservation()[0]
+        st = lbm.surrogate_stats()
+        print("thruster1024:", st, "mass", m0, m1)
+        u = lbm.read_field(Field.U)
+        assert np.isfinite(u).all() and np.isfinite(lbm.read_field(Field.RHO)).all()
+        assert np.abs(u).max() < 0.2
+        # the inlet holds rho and u fixed (equilibrium boundary): the mass grows by
+        # what it feeds in, far below 1 % over a few steps
+        assert abs(m1 / m0 - 1) < 1e-3
+        assert st["surrogate_steps"] >= 4 and st["direct_steps"] >= 2 and st["fallbacks"] == 0
+        assert st["last_residual_e"] < 0.01 and st["last_residual_b"] < 0.01
+        assert np.count_nonzero((flags & 0x1F) == 1) > 0
+    finally:
+        lbm.close()

@@ -298,3 +298,32 @@ def test_charge_inlet_off_conserves_charge():
     for _ in range(200):
         orc.do_time_step()
     assert abs((orc.buf("q") + 1.0).sum() - t0) < 1e-12 * abs(t0)
+
+
+@pytest.mark.parametrize("ft,tol", [(O.FP64, 1e-14), (O.FP32, 1e-6), (O.FP16C, 2e-3)])
+def test_electron_density0_starts_the_electron_gas(ft, tol):
+    """Build extension electron_density0 = n0: initialize writes the electron
+    DDFs of non-solid cells as the textbook equilibrium of density n0 at the
+    cell's velocity (moments n0, n0 u, n0 (u u + I/3) before the -w_i shift),
+    and those of solid cells as density 0 at rest, as the reference does
+    for every cell."""
+    from test_oracle_lorentz import C19, I3, moments, to_t1_layout
+    n0, u0 = 0.75, np.array([0.03, -0.02, 0.01])  # n0 exact in float (the parameter is f32)
+    n = 4
+    orc = O.OracleSim(n, n, n, lod_depth=1, float_type=ft, ext=MHD_EXT, w=1.2, kkge=-1.0, ei_rho0=n0)
+    orc.buf("u")[:] = np.repeat(u0, n ** 3)
+    orc.initialize()
+    e = orc.ddfs("ei").astype(np.float64)
+    assert np.all(e == e[:, :1])  # uniform lattice: every cell's slots alike
+    rho, j, pi = moments(to_t1_layout(e[:, 0]))
+    assert abs(rho - n0) < tol
+    np.testing.assert_allclose(j, n0 * u0, rtol=0, atol=tol)
+    np.testing.assert_allclose(pi, n0 * (np.outer(u0, u0) + I3 / 3) - I3 / 3, rtol=0, atol=tol)
+    assert C19.shape == (3, 19)
+    # solid cells start at density 0: the total over every slot counts fluid cells only
+    orc = O.OracleSim(n, n, n, lod_depth=1, float_type=ft, ext=MHD_EXT, w=1.2, kkge=-1.0, ei_rho0=n0)
+    flags = orc.buf("flags")
+    flags[::5] = 1
+    orc.initialize()
+    total = orc.ddfs("ei").astype(np.float64).sum() + n ** 3
+    assert abs(total - n0 * np.count_nonzero(flags != 1)) < tol * n ** 3
