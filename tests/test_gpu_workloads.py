@@ -8,13 +8,19 @@
   and over 10 steps from the reference's own initial state, in which the
   electron gas starts at the rounding residue of its shifted equilibrium and
   the NaN it produces spreads (NaN positions must match too);
+* the NaN/Inf watchdog stops that run at its first non-finite charge sum;
 * storage formats on the 128^3 plasma blob workload (LOD depth 4, EXACT
   field arithmetic, 50 steps): the relative L2 error of rho - 1, u, E - E_stat,
   B - B_stat and Q of FP32, FP16S and FP16C storage against the FP64
-  instantiation of the same kernels, under the bounds stated below;
+  instantiation of the same kernels, under the bounds stated below (B - B_stat
+  is about 1e-5 of B_stat there, so the f32 rounding of the stored B_dyn,
+  about 2.3e-3 of it, dominates its error in every format);
 * the blob workload at 384^3 (LOD depth 4): FAST field arithmetic within
-  1e-5 of EXACT on identical LOD sums, and gas mass and charge conserved over
-  20 plasma steps in each storage format;
+  1e-5 of EXACT on identical LOD sums after 4 plasma steps (initialize's
+  field update sees no LOD sums, so the first field that moves the gas is
+  step 2's and the LOD velocities are non-zero from step 3: B - B_stat is
+  non-zero), and gas mass and charge conserved over 20 plasma steps in each
+  storage format;
 * the 1024^3 FP16C thruster (quartz tube, ring magnet, charge inlet, 8 blobs,
   plasma background) with the learned field step judged by the direct kernel:
   a few steps, finite state, gas mass within the FP16C rounding drift, the
@@ -68,8 +74,8 @@ def test_mhd_blobs64_bit_exact(plasma, steps):
             if s in (0, steps // 2):
                 assert compare_all(gpu, orc, info) == [], f"step {s + 1}"
         assert compare_all(gpu, orc, info) == []
-        finite = np.isfinite(gpu.read_field(Field.U)).all()
-        assert finite if plasma else not finite  # the reference's start turns NaN within 10 steps
+        finite = np.isfinite(gpu.read_field(Field.U)).all() and np.isfinite(gpu.read_field(Field.Q)).all()
+        assert finite if plasma else not finite  # the reference's start has NaN charges within 10 steps
     finally:
         gpu.close()
 
@@ -101,9 +107,9 @@ def test_watchdog_stops_at_the_first_nonfinite_sum():
 
 # rel-L2 bounds against the FP64 instantiation after 50 steps at 128^3
 FORMAT_BOUNDS = {
-    FloatType.FP32: dict(rho=1e-4, u=1e-4, E=1e-4, B=1e-4, Q=1e-4),
-    FloatType.FP16S: dict(rho=2e-2, u=2e-2, E=5e-2, B=5e-2, Q=5e-2),
-    FloatType.FP16C: dict(rho=2e-2, u=2e-2, E=5e-2, B=5e-2, Q=5e-2),
+    FloatType.FP32: dict(rho=5e-5, u=5e-5, E=5e-5, B=5e-3, Q=5e-5),
+    FloatType.FP16S: dict(rho=1e-3, u=1e-3, E=1e-3, B=5e-3, Q=1e-3),
+    FloatType.FP16C: dict(rho=5e-4, u=5e-4, E=5e-4, B=5e-3, Q=5e-4),
 }
 
 
@@ -141,21 +147,24 @@ def test_storage_formats_against_fp64_mhd128():
 
 
 def test_mhd384_fast_matches_exact():
+    cfg, arrays, _ = scenarios.mhd_blobs(n=384, lod_depth=4, plasma=1.0)
+    lbm = I.Lbm(cfg)
     out = {}
-    for mode in (FieldMode.EXACT, FieldMode.FAST):
-        cfg, arrays, _ = scenarios.mhd_blobs(n=384, lod_depth=4, plasma=1.0)
-        lbm = I.Lbm(cfg)
-        try:
-            scenarios.apply(lbm, arrays)
+    try:
+        scenarios.apply(lbm, arrays)
+        lbm.set_field_mode(FieldMode.EXACT)
+        lbm.initialize()
+        lbm.run(4)  # the LOD velocities are 0 until the first field has moved the gas (step 3)
+        lod = lbm.read_field(Field.QU_LOD)
+        for mode in (FieldMode.EXACT, FieldMode.FAST):
             lbm.set_field_mode(mode)
-            lbm.initialize()
-            lbm.do_time_step()
+            lbm.compute_fields()  # from the same LOD sums
             out[mode] = [lbm.read_field(f) for f in (Field.QU_LOD, Field.E_DYN, Field.B_DYN, Field.E_STAT,
                                                        Field.B_STAT)]
-        finally:
-            lbm.close()
+    finally:
+        lbm.close()
     ex, fa = out[FieldMode.EXACT], out[FieldMode.FAST]
-    assert np.array_equal(ex[0], fa[0])
+    assert np.array_equal(ex[0], lod) and np.array_equal(fa[0], lod)
     assert np.abs(ex[1] - ex[3]).max() > 0 and np.abs(ex[2] - ex[4]).max() > 0
     assert rel_l2(fa[1] - fa[3], ex[1] - ex[3]) < 1e-5
     assert rel_l2(fa[2] - fa[4], ex[2] - ex[4]) < 1e-5
