@@ -220,6 +220,41 @@ int ion_get_timing(ion_handle* h, ion_timing* out);
  * float -> u16 code, dir 1 decodes u16 -> float, for ION_FP16S/ION_FP16C. */
 int ion_device_codec(int float_type, int dir, const void* src, void* dst, uint64_t n);
 
+/* Device buffers of ion_update_e_b_noalloc, in the layouts ion_read_field
+ * uses: flags, q [N]; u, e_stat, b_stat, e_dyn, b_dyn [3][N] (x block, y
+ * block, z block); qu_lod [n_lod][4] (q, vx, vy, vz), as stream_collide
+ * leaves it. q and u are read only when the near-field block holds more than
+ * one cell (n_x / 2^(2^mhd_lod_depth) > 1); they may be NULL otherwise. */
+typedef struct ion_field_buffers {
+    const uint8_t* flags;
+    const float* q;
+    const float* u;
+    const float* qu_lod;
+    const float* e_stat;
+    const float* b_stat;
+    float* e_dyn;
+    float* b_dyn;
+} ion_field_buffers;
+
+/* Most LOD sources ion_update_e_b_noalloc holds: 8^5, so mhd_lod_depth <= 5. */
+#define ION_NOALLOC_MAX_SOURCES 32768u
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The field step of ion_compute_fields
+ * (update_e_b_dynamic, src/kernels/sim_kernels.cl:897-993) on caller-owned
+ * device buffers; E_dyn and B_dyn are bit-identical to ion_compute_fields on
+ * the same inputs. It allocates nothing: the compacted LOD source list lives
+ * in a static device array of ION_NOALLOC_MAX_SOURCES entries (1 MiB), one
+ * per GPU, held until the process ends. cfg supplies the lattice, the LOD
+ * depth (<= 5) and the units (KE, KMU); ext_magneto_hydro, FP32/FP16S/FP16C
+ * (fields are float). field_mode: ION_FIELD_EXACT or ION_FIELD_FAST.
+ * Enqueued on `stream` (a hipStream_t; NULL: the null stream) without
+ * synchronising; on the current device. Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU race, so serialise them (one
+ * stream, or events between streams). */
+int ion_update_e_b_noalloc(const ion_config* cfg, const ion_field_buffers* buf, int field_mode, void* stream);
+
 /* ------------------------------------------------------------------------
  * Learned field surrogate (build-side; the reference has no counterpart).
  * A linear 4-level 3D conv neural operator on the LOD grid (BF16 MFMA, FP32

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -815,6 +816,104 @@ int ion_compute_fields(ion_handle* h) {
     int rc = field_update(h);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
+    return ION_OK;
+}
+
+}  // extern "C"
+
+// Scratch of ion_update_e_b_noalloc: the compacted LOD sources (centre and
+// charge, then mean velocity and LOD index: ION_NOALLOC_MAX_SOURCES float4
+// each) followed by their count. Every GPU the library runs on holds its own
+// copy at its own address; host code uses that address (scratch_on_device),
+// never the array's name, which in host code is only a stand-in.
+static __device__ __align__(256) float4 scratch[2 * ION_NOALLOC_MAX_SOURCES + 16];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+int scratch_on_device(float4** out) {
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(ION_ERR_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(ION_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned no address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = (float4*)g_scratch_ptr[dev];
+    return ION_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ion_update_e_b_noalloc(const ion_config* cfg, const ion_field_buffers* b, int field_mode, void* stream) {
+    if (!cfg || !b) return fail(ION_ERR_INVALID, "null argument");
+    int rc = validate(*cfg);
+    if (rc) return rc;
+    if (!cfg->ext_magneto_hydro) return fail(ION_ERR_INVALID, "fields need ext_magneto_hydro");
+    if (cfg->float_type == ION_FP64)
+        return fail(ION_ERR_UNSUPPORTED, "ion_update_e_b_noalloc takes float fields (FP32, FP16S or FP16C lattices)");
+    if (field_mode != ION_FIELD_EXACT && field_mode != ION_FIELD_FAST) return fail(ION_ERR_INVALID, "invalid field mode");
+    const uint32_t D = cfg->mhd_lod_depth;
+    const uint64_t n_top = 1ull << (3 * D);
+    if (n_top > ION_NOALLOC_MAX_SOURCES)
+        return fail(ION_ERR_UNSUPPORTED, "mhd_lod_depth " + std::to_string(D) + ": " + std::to_string(n_top) +
+                                             " LOD sources, more than the static scratch array holds (" +
+                                             std::to_string(ION_NOALLOC_MAX_SOURCES) + ", depth 5)");
+    if (!b->flags || !b->qu_lod || !b->e_stat || !b->b_stat || !b->e_dyn || !b->b_dyn)
+        return fail(ION_ERR_INVALID, "null device buffer");
+    ion::FieldArgs f{};
+    const uint32_t nd = 1u << D;
+    // near-field block side imax(N/(1<<nd), 1), as in field_update
+    const uint32_t sh = 1u << (nd & 31u);
+    auto side = [&](uint32_t n) { uint32_t v = n / sh; return v < 1 ? 1u : v; };
+    f.dsx = side(cfg->n_x);
+    f.dsy = side(cfg->n_y);
+    f.dsz = side(cfg->n_z);
+    f.near = ((uint64_t)f.dsx * f.dsy * f.dsz > 1) ? 1u : 0u;
+    if (f.near && (!b->q || !b->u)) return fail(ION_ERR_INVALID, "the near-field block needs the q and u buffers");
+    float4* s = nullptr;
+    if ((rc = scratch_on_device(&s))) return rc;
+    ion_kernel_constants k;
+    ion_kernel_constants_of(cfg, &k);
+    const uint32_t hi = (uint32_t)n_lod_own_of(D), lo = (uint32_t)(hi - n_top);
+    float4* src_pq = s;
+    float4* src_vd = s + ION_NOALLOC_MAX_SOURCES;
+    uint32_t* count = (uint32_t*)(s + 2 * ION_NOALLOC_MAX_SOURCES);
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = ion::launch_prep_sources(b->qu_lod, false, lo, hi, cfg->n_x, cfg->n_y, cfg->n_z, D, src_pq, src_vd,
+                                            count, st);
+    if (e != hipSuccess) return fail(ION_ERR_DEVICE, std::string("source compaction: ") + hipGetErrorString(e));
+    f.e_stat = b->e_stat;
+    f.b_stat = b->b_stat;
+    f.e_dyn = b->e_dyn;
+    f.b_dyn = b->b_dyn;
+    f.flags = b->flags;
+    f.q = b->q;
+    f.u = b->u;
+    f.src_pq = src_pq;
+    f.src_vd = src_vd;
+    f.src_count = count;
+    f.nx = cfg->n_x;
+    f.ny = cfg->n_y;
+    f.nz = cfg->n_z;
+    f.D = D;
+    f.N = (uint64_t)cfg->n_x * cfg->n_y * cfg->n_z;
+    f.ke = k.ke;
+    f.kmu = k.kmu;
+    f.raw = nullptr;
+    f.f64 = 0u;
+    e = ion::launch_update_e_b(f, field_mode == ION_FIELD_FAST, st);
+    if (e != hipSuccess) return fail(ION_ERR_DEVICE, std::string("field kernel: ") + hipGetErrorString(e));
     return ION_OK;
 }
 

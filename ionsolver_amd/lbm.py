@@ -170,6 +170,7 @@ ABI = {
     "ion_set_timing": (_I, [_VP, _I]),
     "ion_get_timing": (_I, [_VP, ctypes.POINTER(_Timing)]),
     "ion_device_codec": (_I, [_I, _I, _VP, _VP, _U64]),
+    "ion_update_e_b_noalloc": (_I, [_VP, _VP, _I, _VP]),
     "ion_set_watchdog": (_I, [_VP, ctypes.c_uint32]),
     "ion_watchdog_status": (_I, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "ion_last_error": (ctypes.c_char_p, []),

@@ -122,3 +122,15 @@ def test_load_errors(tmp_path):
     p.write_bytes(b"IonSolver setup\n" + bytes(10))
     with pytest.raises(I.IonError, match=r"\[-5\]"):
         I.Lbm.load(str(p))
+
+
+def test_no_alloc_field_step_rejects_depth_beyond_its_scratch_array():
+    """ion_update_e_b_noalloc holds at most 8^5 LOD sources; the depth check
+    runs before any device work, so the buffer pointers are never used."""
+    import ctypes
+    from harness import make_config
+    cfg = make_config(nx=64, ny=64, nz=64, mhd=True, lod_depth=6, float_type=2)
+    buf = (ctypes.c_void_p * 8)(*([1] * 8))
+    lib = I.load_library()
+    assert lib.ion_update_e_b_noalloc(ctypes.byref(cfg._to_c()), buf, 1, None) != 0
+    assert "scratch" in lib.ion_last_error().decode()
