@@ -167,41 +167,99 @@ def static_field_leg(scale=1.0):
             "arithmetic": "the reference's: unfused float, correctly rounded sqrt and division, same summation order"}
 
 
-def surrogate_leg(lbm, cfg, samples=4, reps=10, channels=16, train_steps=2000, lr=1e-4, lr_head=1e-5):
+def zero_unet_columns(lbm, cfg, channels):
+    """The operator without the U-Net: the head's feature columns (the
+    network's contribution) cleared, its direct-source columns kept. Head
+    rows: 6 field components x cells per LOD block; columns: 27 block
+    neighbours x (c0 feature + 8 source channels), padded to 32."""
+    p = lbm.surrogate_params()
+    nd = 1 << cfg.mhd_lod_depth
+    nout = 6 * (cfg.n_x // nd) * (cfg.n_y // nd) * (cfg.n_z // nd)
+    hcin = channels + 8
+    hk = (27 * hcin + 31) // 32 * 32
+    head = p[-nout * hk:].reshape(nout, hk)
+    for d in range(27):
+        head[:, d * hcin:d * hcin + channels] = 0.0
+    lbm.set_surrogate_params(p)
+
+
+def surrogate_leg(lbm, cfg, workload_name, samples=4, reps=10, channels=16, train_steps=2000, lr=1e-4, lr_head=1e-5):
     """Learned field surrogate against the direct field step on the same
-    lattice. Accuracy: the surrogate's relative L2 residual of the dynamic E
-    and B against the direct kernel's raw sums, on `samples` random charge
-    configurations held out from training (max over samples), before and
-    after `train_steps` GPU training steps on freshly generated
-    configurations (Adam, the rate decaying linearly over the last 40 %).
-    Speed: field-step time of both solvers (library timing slots: source
-    compaction + field kernel(s), HIP events on the library stream)."""
+    lattice. Training (GPU, Adam, the rate decaying linearly over the last
+    40 %): `train_steps` steps, alternating fresh random 8-blob LOD
+    configurations and snapshots of an FP16C trajectory of the same workload
+    (plasma background, advanced one step per snapshot), direct-kernel
+    targets. Accuracy: relative L2 residual of the dynamic E and B against the
+    direct kernel's raw sums, held out from training: `samples` random
+    configurations and `samples` snapshots of a trajectory of another blob
+    seed (max over each set), for the untrained network, the trained one and
+    the trained one without its U-Net (feature columns zeroed). Speed:
+    field-step time of both solvers (library timing slots: source compaction
+    + field kernel(s), HIP events on the library stream)."""
+    import ionsolver_amd as I
+    from ionsolver_amd import Field, FieldMode, FloatType, scenarios
     from ionsolver_amd.scenarios import random_lod
     from ionsolver_amd.surrogate import lod_buffer
-    from ionsolver_amd import Field
     lbm.surrogate_create(channels=channels)
     shape, D = (cfg.n_x, cfg.n_y, cfg.n_z), cfg.mhd_lod_depth
+    plasma = None if workload_name.endswith("-residue") else 1.0
 
-    def accuracy():
-        rng = np.random.default_rng(777)
-        res = []
-        for _ in range(samples):
-            lbm.write_field(Field.QU_LOD, lod_buffer(lbm, random_lod(shape, D, rng)))
-            lbm.compute_fields()
-            res.append(lbm.surrogate_residual())
-        return np.array(res)
+    def trajectory(seed):
+        c, arrays, _ = scenarios.mhd_blobs(n=cfg.n_x, float_type=FloatType.FP16C, plasma=plasma, seed=seed)
+        t = I.Lbm(c)
+        scenarios.apply(t, arrays)
+        t.set_field_mode(FieldMode.FAST)
+        t.initialize()
+        return t
 
-    res0 = accuracy()
+    # trajectories start after a warm-up of 20 steps: the gas (and so the
+    # current J = q v behind B) only starts moving once the first fields act
+    held = trajectory(4321)
+    held.run(20)
+    snaps = []
+    for _ in range(samples):
+        held.run(10)
+        snaps.append(held.read_field(Field.QU_LOD))
+    held.close()
+    rng = np.random.default_rng(777)
+    randoms = [lod_buffer(lbm, random_lod(shape, D, rng)) for _ in range(samples)]
+
+    def accuracy(sets):
+        out = []
+        for lods in sets:
+            res = []
+            for lod in lods:
+                lbm.write_field(Field.QU_LOD, lod)
+                lbm.compute_fields()
+                res.append(lbm.surrogate_residual())
+            out.append(np.array(res).max(0))
+        return out
+
+    res0 = accuracy((randoms, snaps))
     rng = np.random.default_rng(1)
+    traj, n_traj, skipped = None, 0, 0
     t0 = time.perf_counter()
     for i in range(train_steps):
         sc = min(1.0, max(0.05, (1.0 - i / train_steps) / 0.4))
         lbm.surrogate_set_learning_rate(lr * sc, lr_head * sc)
-        lbm.write_field(Field.QU_LOD, lod_buffer(lbm, random_lod(shape, D, rng)))
+        if i % 2:
+            if traj is None or traj.get_time_step() >= 70:  # a new blob seed every 50 snapshots
+                if traj is not None:
+                    traj.close()
+                traj = trajectory(1234 + n_traj)
+                traj.run(20)
+                n_traj += 1
+            traj.do_time_step()
+            lbm.write_field(Field.QU_LOD, traj.read_field(Field.QU_LOD))
+        else:
+            lbm.write_field(Field.QU_LOD, lod_buffer(lbm, random_lod(shape, D, rng)))
         lbm.compute_fields()
-        lbm.surrogate_train_step(True)
+        re_, rb_ = lbm.surrogate_train_step(True)
+        skipped += not (np.isfinite(re_) and np.isfinite(rb_))
     train_s = time.perf_counter() - t0
-    res = accuracy() if train_steps else res0
+    if traj is not None:
+        traj.close()
+    res = accuracy((randoms, snaps))
 
     def field_ms(fn):
         fn()
@@ -213,13 +271,23 @@ def surrogate_leg(lbm, cfg, samples=4, reps=10, channels=16, train_steps=2000, l
         return (t["ms_field_prep"] + t["ms_update_e_b"]) / reps
 
     d_ms, s_ms = field_ms(lbm.compute_fields), field_ms(lbm.surrogate_infer)
+    zero_unet_columns(lbm, cfg, channels)
+    res_wo = accuracy((randoms, snaps))
+    r = lambda v: [float(v[0]), float(v[1])]
     return {"field_step_ms_direct": round(d_ms, 4), "field_step_ms_surrogate": round(s_ms, 4),
-            "speedup": round(d_ms / s_ms, 2), "rel_l2_e": float(res[:, 0].max()), "rel_l2_b": float(res[:, 1].max()),
-            "rel_l2_e_untrained": float(res0[:, 0].max()), "rel_l2_b_untrained": float(res0[:, 1].max()),
+            "speedup": round(d_ms / s_ms, 2),
+            "rel_l2_e": float(max(res[0][0], res[1][0])), "rel_l2_b": float(max(res[0][1], res[1][1])),
+            "held_out": {"random_configs": {"trained": r(res[0]), "untrained": r(res0[0]), "trained_without_unet": r(res_wo[0])},
+                         "trajectory_snapshots": {"trained": r(res[1]), "untrained": r(res0[1]),
+                                                  "trained_without_unet": r(res_wo[1])},
+                         "note": "(E, B) relative L2 of the dynamic parts, max over the set"},
             "channels": channels, "train_steps": train_steps, "train_s": round(train_s, 2),
             "train_ms_per_step": round(1e3 * train_s / max(train_steps, 1), 2),
-            "accuracy_sample": f"{samples} held-out random 8-blob LOD configurations on this lattice (max)",
-            "training_data": "fresh random 8-blob LOD configurations per step, direct-kernel targets"}
+            "accuracy_sample": f"{samples} held-out random 8-blob LOD configurations and {samples} snapshots "
+                               f"(steps 30, 40, ...) of an FP16C trajectory with another blob seed",
+            "training_data": f"alternating fresh random 8-blob LOD configurations and snapshots of {n_traj} "
+                             f"FP16C trajectories (blob seeds 1234, 1235, ...; steps 21 to 70 of each), "
+                             f"direct-kernel targets; {skipped} samples with a non-finite loss skipped"}
 
 
 def rel_l2(a, b):
@@ -342,10 +410,14 @@ def fp64_error(name, steps, field_mode, n=128):
             "rel_l2": {k: rel_l2(got[k], ref[k]) for k in ref}}
 
 
-def measure(name, steps, warmup, field_mode, thruster=False):
+def measure(name, steps, warmup, field_mode, thruster=False, train_steps=0):
     """Set up a workload, run `warmup` steps, time `steps` steps (host clock
     around run(), per-kernel HIP events on the library stream). Returns the
-    open Lbm and the measurements."""
+    open Lbm and the measurements. The thruster runs on the learned field
+    step judged by the direct kernel every 50 field steps; with train_steps,
+    its network is first trained on the GPU on the thruster's own first
+    train_steps steps (each a direct field step whose raw sums are the
+    target), and the timed run continues from there."""
     import ionsolver_amd as I
     from ionsolver_amd import scenarios
     cfg, arrays, meta = workload(name)
@@ -354,11 +426,24 @@ def measure(name, steps, warmup, field_mode, thruster=False):
     flags = arrays.get("flags")
     del arrays
     lbm.set_field_mode(field_mode)
+    train_info = None
     if thruster:
         # learned field step, the direct kernel every 50 field steps as judge
-        lbm.surrogate_create(channels=16)
+        lbm.surrogate_create(channels=16, lr=1e-4, lr_head=1e-5)
+        if train_steps:
+            lbm.initialize()
+            t0 = time.perf_counter()
+            for i in range(train_steps):
+                lbm.do_time_step()  # direct field step: the raw sums of this state are the target
+                lbm.surrogate_train_step(True)
+                if i % 20 == 19:
+                    print(f"thruster surrogate training {i + 1}/{train_steps}: {time.perf_counter() - t0:.1f} s",
+                          file=sys.stderr, flush=True)
+            train_info = {"train_steps": train_steps, "train_s": round(time.perf_counter() - t0, 1),
+                          "data": "the run's own first steps (direct field steps), direct-kernel targets"}
         lbm.set_field_solver(I.FieldSolver.SURROGATE, judge_every=50, tolerance=0.01)
-    lbm.initialize()
+    if not train_info:
+        lbm.initialize()
     if warmup:
         lbm.run(warmup)
     lbm.set_timing(True)
@@ -384,22 +469,27 @@ def measure(name, steps, warmup, field_mode, thruster=False):
           "GB/s": round(sc_bytes / sc_ms / 1e6, 1) if sc_ms else None,
           "frac_hbm": round(sc_bytes / sc_ms / 1e6 / HBM_PEAK_GBS, 4) if sc_ms else None}
     return lbm, dict(cfg=cfg, meta=meta, flags=flags, N=N, n_f=n_f, dt=dt, tm=tm,
-                     mlups=N * steps / dt / 1e6, sc=sc)
+                     mlups=N * steps / dt / 1e6, sc=sc, train=train_info)
 
 
-def field_interactions(cfg, flags):
+def field_interactions(cfg, flags, qu_lod):
     """Interactions of one direct field step: fluid cells x top-level LOD
-    sources inside the range the reference reads, less each cell's own LOD."""
+    sources inside the range the reference reads that carry charge (a source
+    of zero charge adds exactly zero; the kernel drops it), less each cell's
+    own LOD. qu_lod: the LOD sums the step used."""
     D = cfg.mhd_lod_depth
     nd = 1 << D
     n_top = nd ** 3
-    lo = ((1 << (3 * (D + 1))) - 1) // 7 - n_top
+    hi = ((1 << (3 * (D + 1))) - 1) // 7
+    lo = hi - n_top
+    live = np.zeros(hi, bool)
+    live[lo:hi] = qu_lod.reshape(-1, 4)[lo:hi, 0] != 0
     N = cfg.n_x * cfg.n_y * cfg.n_z
     z, rem = np.divmod(np.arange(N, dtype=np.int64), cfg.n_x * cfg.n_y)
     y, x = np.divmod(rem, cfg.n_x)
     own = x // (cfg.n_x // nd) + (y // (cfg.n_y // nd) + z // (cfg.n_z // nd) * nd) * nd
     fl = (flags & 0x1F) != 1
-    return int(np.count_nonzero(fl)) * (n_top - lo) - int(np.sum(fl & (own >= lo)))
+    return int(np.count_nonzero(fl)) * int(live.sum()) - int(np.sum(fl & live[own]))
 
 
 def probes(out, kernels, cfg, N):
@@ -461,8 +551,9 @@ def main():
     ap.add_argument("--field-mode", default="fast", choices=["exact", "fast"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned field surrogate leg")
-    ap.add_argument("--surrogate-train", type=int, default=2000,
-                    help="GPU training steps (fresh snapshots) of the surrogate leg")
+    ap.add_argument("--surrogate-train", type=int, default=None,
+                    help="GPU training steps of the surrogate (default: 2000 for the surrogate leg of the MHD "
+                         "workloads, 0 for the thruster)")
     ap.add_argument("--no-fp16c", action="store_true", help="skip the FP16C storage leg of an FP32 MHD workload")
     ap.add_argument("--no-fp64-error", action="store_true",
                     help="skip the error of the measured configuration against the FP64 instantiation")
@@ -476,6 +567,8 @@ def main():
                     help="first run the HBM copy probe (3 copies of 1 GiB: reads 1 GiB, writes 1 GiB each): "
                          "the known-size kernel for PMC FETCH_SIZE/WRITE_SIZE scale factors")
     args = ap.parse_args()
+    if args.surrogate_train is None:
+        args.surrogate_train = 0 if args.workload.startswith("thruster") else 2000
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
                           "n_gpus": args.gpus}))
@@ -494,7 +587,8 @@ def main():
         I.hbm_copy_probe(1 << 30, 2)
     thruster = args.workload.startswith("thruster")
     mode = FieldMode.FAST if args.field_mode == "fast" else FieldMode.EXACT
-    lbm, m = measure(args.workload, args.steps, args.warmup, mode, thruster=thruster)
+    lbm, m = measure(args.workload, args.steps, args.warmup, mode, thruster=thruster,
+                     train_steps=args.surrogate_train if thruster else 0)
     cfg, tm, k, N, n_f = m["cfg"], m["tm"], args.steps, m["N"], m["n_f"]
     state = state_check(lbm, cfg)
     kernels = {"stream_collide": m["sc"]}
@@ -504,7 +598,8 @@ def main():
         roofline = {"bound": "hbm", "achieved": m["sc"]["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": m["sc"]["frac_hbm"], "traffic": None, "kernel": "stream_collide"}
     elif cfg.ext_magneto_hydro:
-        interactions = field_interactions(cfg, m["flags"])
+        from ionsolver_amd import Field
+        interactions = field_interactions(cfg, m["flags"], lbm.read_field(Field.QU_LOD))
         eb_ms = tm["ms_update_e_b"] / k
         eb_tflops = interactions * FLOP_PER_INTERACTION / (eb_ms * 1e-3) / 1e12
         kernels["update_e_b_dynamic"] = {"ms": round(eb_ms, 4), "interactions": interactions,
@@ -539,8 +634,9 @@ def main():
         out["surrogate"] = {k2: v for k2, v in lbm.surrogate_stats().items()}
         out["surrogate"]["note"] = ("field steps of the whole run including warm-up; the first is the direct "
                                     "kernel (judge), whose residual is last_residual_e/b")
+        out["surrogate"]["training"] = m["train"]
     elif cfg.ext_magneto_hydro and not args.no_surrogate and cfg.mhd_lod_depth >= 3:
-        out["surrogate"] = surrogate_leg(lbm, cfg, train_steps=args.surrogate_train)
+        out["surrogate"] = surrogate_leg(lbm, cfg, args.workload, train_steps=args.surrogate_train)
     lbm.close()
     if cfg.ext_magneto_hydro and not thruster and int(cfg.float_type) == 2 and not args.no_fp16c:
         parts = args.workload.split("-")

@@ -299,8 +299,12 @@ __global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b_f64(FieldArgs a) {
 // specialised correctly rounded operators of cr_math.hpp: |r|^2 is a
 // multiple of 1/4 in [1/4, 2^24) (cell minus block centre), |r|^3 lies in
 // [1/8, 2^36] and the dividends r_i are 0 or in [1/2, 2^11].
+// Held to 80 VGPRs (6 wavefronts per SIMD, no spills; the sweep's VALU
+// instructions are the same as at 105 VGPRs) so that the scalar loads at the
+// top of each source's iteration hide behind the other wavefronts.
 template <int ROW, bool FAST>
-__global__ void __launch_bounds__(256) k_update_e_b_tiled(FieldArgs a, uint32_t TX, uint32_t TY, uint32_t TZ) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+k_update_e_b_tiled(FieldArgs a, uint32_t TX, uint32_t TY, uint32_t TZ) {
     __shared__ float s_epi[ROW * 256];
     const uint32_t NX = a.nx, NY = a.ny;
     const uint32_t tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);

@@ -329,8 +329,13 @@ int Net::loss(const Fields& f, double* rel_e, double* rel_b) {
     double s[4] = {0, 0, 0, 0};
     for (size_t i = 0; i < n_partials_; i++)
         for (int k = 0; k < 4; k++) s[k] += p[i * 4 + k];
-    norm_host_[0] = (float)s[1];
-    norm_host_[1] = (float)s[3];
+    // The loss weights each group by 1 / sum t^2. A group whose targets are
+    // negligible against the other's (a flow that has barely started moving:
+    // J = q v close to 0, so B far below E) would get a weight that
+    // overflows the bf16 gradients; it is left out of that step's gradient.
+    constexpr double NEGLIGIBLE = 1e-8;
+    norm_host_[0] = s[1] > NEGLIGIBLE * s[3] ? (float)s[1] : 0.0f;
+    norm_host_[1] = s[3] > NEGLIGIBLE * s[1] ? (float)s[3] : 0.0f;
     if (rel_e) *rel_e = s[1] > 0 ? std::sqrt(s[0] / s[1]) : 0.0;
     if (rel_b) *rel_b = s[3] > 0 ? std::sqrt(s[2] / s[3]) : 0.0;
     return 0;
@@ -408,7 +413,13 @@ int Net::train_step(const Fields& f, bool update, double* rel_e, double* rel_b) 
         (rc = far_field(f)))
         return rc;
     SCHK(gemm_nt(EPI_HEAD, hcol_, hw_, (int)nblk_, (int)nout_, hk_, hk_, hk_, head_out(f, y_), stream_));
-    if ((rc = loss(f, rel_e, rel_b))) return rc;
+    double re = 0.0, rb = 0.0;
+    if ((rc = loss(f, &re, &rb))) return rc;
+    if (rel_e) *rel_e = re;
+    if (rel_b) *rel_b = rb;
+    // a sample whose loss is not finite (a target or an output that overflowed)
+    // would poison the Adam moments: no gradient step for it
+    if (!std::isfinite(re) || !std::isfinite(rb)) return 0;
     SCHK(hipMemcpyAsync(norms_, norm_host_, 2 * sizeof(float), hipMemcpyHostToDevice, stream_));
     HeadGeom hg{ncell_, geo_.nx, geo_.ny, geo_.nz, nd_, geo_.nx / nd_, geo_.ny / nd_, geo_.nz / nd_, cpb_,
                 (uint32_t)nout_pad_, (uint32_t)nblk_pad_};

@@ -71,10 +71,11 @@ def mhd_blobs(n=256, seed=1234, float_type=FloatType.FP32, lod_depth=4, n_blobs=
     Blob centres are uniform in [2 sigma, n - 2 sigma)^3 outside the
     cylinder; sign +-1 and amplitude U[0.5, 1]*amp come from the same seeded
     stream. With plasma = n0 the blobs sit on a neutral background: the ion
-    charge density written to Q (the charge DDFs' density) is n0 + blobs and
-    the electron gas starts at density n0 (use_plasma_units), so that the cell
-    charge Q = rho_q - rho_e is the blobs; otherwise (the reference's initial
-    state) the charge density is the blobs and the electron gas starts at 0.
+    charge density written to Q (the charge DDFs' density) is n0 + blobs +
+    1e-4 normal noise (same stream) and the electron gas starts at density n0
+    (use_plasma_units), so that the cell charge Q = rho_q - rho_e is the blobs
+    plus the noise; otherwise (the reference's initial state) the charge
+    density is the blobs and the electron gas starts at 0.
     Returns (cfg, arrays, blobs)."""
     nx, ny, nz = shape if shape is not None else (n, n, n)
     cfg = LbmConfig()
@@ -111,7 +112,10 @@ def mhd_blobs(n=256, seed=1234, float_type=FloatType.FP32, lod_depth=4, n_blobs=
         q += a * np.exp(-((x - bx) ** 2 + (y - by) ** 2 + (z - bz) ** 2) / (2 * sigma * sigma))
     if plasma is not None:
         use_plasma_units(cfg, plasma)
-        q += plasma
+        # ion density fluctuations of 1e-4 around the background: every LOD
+        # block carries charge from the first step, as it does once the
+        # electron gas has moved
+        q += plasma + 1e-4 * rng.standard_normal(q.size)
     q[solid] = 0.0
     # B_stat = B0 z + point dipole m z at the domain centre: (3 (m.r^) r^ - m) / |r|^3
     rx, ry, rz = x - cx, y - cy, z - 0.5 * nz

@@ -228,7 +228,9 @@ def far_interp(geo, samples):
 
 def loss_and_grad(geo, params, qu, T, fluid, y_add=None):
     """Loss sum_E (y-t)^2/sum_E t^2 + sum_B (y-t)^2/sum_B t^2 over fluid cells,
-    its gradient w.r.t. the flat parameters, and (rel_e, rel_b). y_add: a
+    its gradient w.r.t. the flat parameters, and (rel_e, rel_b). A group whose
+    sum t^2 is at most 1e-8 of the other's is left out of the loss and the
+    gradient (its weight would overflow the GPU's bf16 gradients). y_add: a
     constant (6, N) term added to the network output (the far field in
     network units)."""
     Y, s, a = forward(geo, params, qu)
@@ -238,8 +240,10 @@ def loss_and_grad(geo, params, qu, T, fluid, y_add=None):
     m = fluid[None, :]
     d = (y - T) * m
     nE, nB = np.sum((T[:3] * m) ** 2), np.sum((T[3:] * m) ** 2)
-    L = np.sum(d[:3] ** 2) / nE + np.sum(d[3:] ** 2) / nB
-    dy = np.concatenate([2 * d[:3] / nE, 2 * d[3:] / nB], 0)
+    wE = 1.0 / nE if nE > 1e-8 * nB else 0.0
+    wB = 1.0 / nB if nB > 1e-8 * nE else 0.0
+    L = np.sum(d[:3] ** 2) * wE + np.sum(d[3:] ** 2) * wB
+    dy = np.concatenate([2 * d[:3] * wE, 2 * d[3:] * wB], 0)
     dY = cells_to_block(geo, dy)
     Ws = geo.split(params)
     W = dict(zip(LAYERS, Ws[:7]))

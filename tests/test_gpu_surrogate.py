@@ -247,3 +247,88 @@ def test_thruster_runs_on_the_surrogate_with_its_judge():
     assert np.all(np.isfinite(out["surrogate"][0]))
     assert rel_l2(out["surrogate"][0], out["direct"][0]) < 1e-3
     assert rel_l2(out["surrogate"][1] - 1, out["direct"][1] - 1) < 5e-3
+
+
+def _zero_feature_columns(lbm, geo):
+    """The same operator without the U-Net: the head's feature columns
+    (the network's contribution) cleared, its direct-source columns kept."""
+    p = lbm.surrogate_params()
+    head = p[-geo.nout * geo.hk:].reshape(geo.nout, geo.hk)
+    for d in range(27):
+        head[:, d * geo.hcin:d * geo.hcin + geo.c0] = 0.0
+    lbm.set_surrogate_params(p)
+
+
+def test_trained_network_gains_over_the_operator_without_it():
+    """LOD depth 4 (the benchmark's) on 128^3, FP16C plasma blob workload:
+    trained on the GPU on random LOD configurations and on snapshots of
+    plasma trajectories (a new blob seed every 50 snapshots), the network's
+    held-out residual (a random configuration set and a trajectory of another
+    blob seed) is below that of
+    the same operator with the U-Net's contribution zeroed: by at least 20 %
+    for E (measured: 46 %). B's far field is smoother and its interpolation
+    leaves little for the network to learn: its residual stays within 15 % of
+    the operator's without the network (measured: 9 % above it, 0.17 %
+    against 0.16 %, far inside the judge's 1 %). Trajectories are sampled
+    after 20 steps, once the fields have set the gas moving."""
+    from ionsolver_amd import FloatType
+    from ionsolver_amd.scenarios import apply, mhd_blobs
+    n, D, c0 = 128, 4, 16
+
+    def sim(seed, surrogate):
+        cfg, arrays, _ = mhd_blobs(n=n, lod_depth=D, float_type=FloatType.FP16C, plasma=1.0, seed=seed)
+        lbm = I.Lbm(cfg)
+        apply(lbm, arrays)
+        lbm.set_field_mode(I.FieldMode.FAST)
+        if surrogate:
+            lbm.surrogate_create(channels=c0, lr=1e-4, lr_head=1e-5)
+        lbm.initialize()
+        return lbm
+
+    trainer, traj, held = sim(1234, True), None, sim(99, False)
+    try:
+        held.run(20)  # the gas (and the current behind B) moves once the first fields act
+        rng = np.random.default_rng(5)
+        shape = (n, n, n)
+        n_traj = 0
+        for i in range(600):
+            if i % 2:
+                if traj is None or traj.get_time_step() >= 70:  # a new blob seed every 50 snapshots
+                    if traj is not None:
+                        traj.close()
+                    traj = sim(1234 + n_traj, False)
+                    traj.run(20)
+                    n_traj += 1
+                traj.do_time_step()
+                lod = traj.read_field(Field.QU_LOD)
+            else:
+                lod = lod_buffer(trainer, random_lod(shape, D, rng))
+            trainer.write_field(Field.QU_LOD, lod)
+            trainer.compute_fields()
+            trainer.surrogate_train_step(True)
+        snaps = []
+        for _ in range(3):
+            held.run(5)
+            snaps.append(held.read_field(Field.QU_LOD))
+        rng = np.random.default_rng(777)
+        snaps += [lod_buffer(trainer, random_lod(shape, D, rng)) for _ in range(3)]
+
+        def residuals():
+            out = []
+            for lod in snaps:
+                trainer.write_field(Field.QU_LOD, lod)
+                trainer.compute_fields()
+                out.append(trainer.surrogate_residual())
+            return np.array(out)
+
+        trained = residuals()
+        _zero_feature_columns(trainer, S.Geometry(n, n, n, D, c0))
+        without = residuals()
+        print("held-out residuals (E, B): trained", trained.mean(0), "without the U-Net", without.mean(0))
+        assert np.all(trained < 1e-2)
+        assert trained[:, 0].mean() < 0.8 * without[:, 0].mean()
+        assert trained[:, 1].mean() < 1.15 * without[:, 1].mean()
+    finally:
+        for lbm in (trainer, traj, held):
+            if lbm is not None:
+                lbm.close()
