@@ -68,7 +68,15 @@ struct FieldArgs {
     float ke, kmu;
     float* raw;             // if set: the raw source sums (E then B, 6 x N) of fluid cells instead of E_dyn / B_dyn
     uint32_t f64;           // FP64 instantiation: every field and source array holds doubles
+    // judge: E_dyn / B_dyn hold the surrogate's raw sums for the fluid cells on
+    // entry; the kernel adds (surrogate - direct)^2 and direct^2 of the E and
+    // the B components into judge[block][4] before writing its own fields
+    double* judge;
 };
+// Number of blocks (judge partials) the field kernel launches for a.
+uint32_t field_blocks(const FieldArgs& a, bool fast);
+// Sums judge partials [nblocks][4] in a fixed order into out[4] (E err, E norm, B err, B norm).
+hipError_t launch_judge_reduce(const double* partial, uint32_t nblocks, double* out, hipStream_t s);
 
 // kernels_lbm.hip
 // ecr: the SUBGRID_ECR variants (D3Q19, MHD, FP32/FP16S/FP16C)

@@ -107,6 +107,29 @@ __device__ __forceinline__ void interact_exact(R q, R vx, R vy, R vz, R rx, R ry
     bz = bz + q * crz;
 }
 
+// Judge partials of a block: the work-items' four sums (E error, E norm,
+// B error, B norm) reduced in a fixed order (wavefront tree, then the
+// wavefronts in order) into partial[blockIdx.x][4], in double.
+__device__ __forceinline__ void judge_block_sums(const float v[4], double* partial) {
+    __shared__ double jsh[16][4];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    double d[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        d[k] = (double)v[k];
+        for (int o = 32; o >= 1; o >>= 1) d[k] += __shfl_down(d[k], o, 64);
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 4; k++) jsh[wv][k] = d[k];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (uint32_t w = 0; w < (blockDim.x + 63u) / 64u; w++) t += jsh[w][threadIdx.x];
+        partial[(size_t)blockIdx.x * 4 + threadIdx.x] = t;
+    }
+}
+
 template <bool FAST, bool NEAR, int CPT>
 __global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b(FieldArgs a) {
     __shared__ float4 s_pq[FIELD_TILE];
@@ -191,6 +214,7 @@ __global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b(FieldArgs a) {
     }
 
     const uint64_t N = a.N;
+    float jd[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int k = 0; k < CPT; k++) {
         if (!act[k]) continue;
@@ -205,6 +229,15 @@ __global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b(FieldArgs a) {
             r[5 * N + n] = bz[k];
             continue;
         }
+        if (a.judge != nullptr) {
+            const float d[6] = {ex[k], ey[k], ez[k], bx[k], by[k], bz[k]};
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+                const float sur = (c < 3 ? a.e_dyn : a.b_dyn)[(uint64_t)(c % 3) * N + n];
+                jd[c < 3 ? 0 : 2] = __builtin_fmaf(sur - d[c], sur - d[c], jd[c < 3 ? 0 : 2]);
+                jd[c < 3 ? 1 : 3] = __builtin_fmaf(d[c], d[c], jd[c < 3 ? 1 : 3]);
+            }
+        }
         a.e_dyn[n] = a.e_stat[n] + a.ke * ex[k];
         a.e_dyn[N + n] = a.e_stat[N + n] + a.ke * ey[k];
         a.e_dyn[2 * N + n] = a.e_stat[2 * N + n] + a.ke * ez[k];
@@ -212,6 +245,7 @@ __global__ void __launch_bounds__(FIELD_BLOCK) k_update_e_b(FieldArgs a) {
         a.b_dyn[N + n] = a.b_stat[N + n] + a.kmu * by[k];
         a.b_dyn[2 * N + n] = a.b_stat[2 * N + n] + a.kmu * bz[k];
     }
+    if (a.judge != nullptr) judge_block_sums(jd, a.judge);
 }
 
 // FP64 instantiation of the field update (the error reference of the
@@ -415,6 +449,7 @@ k_update_e_b_tiled(FieldArgs a, uint32_t TX, uint32_t TY, uint32_t TZ) {
         gn[j] = (uint64_t)(bx * W + lx) + ((uint64_t)(by * TY + ly) + (uint64_t)(bz * TZ + lz) * NY) * NX;
         if ((a.flags[gn[j]] & 0x1F) != 0x01) live |= 1u << j;
     }
+    float jd[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int c = 0; c < 6; c++) {
         __syncthreads();
@@ -432,13 +467,19 @@ k_update_e_b_tiled(FieldArgs a, uint32_t TX, uint32_t TY, uint32_t TZ) {
             const uint64_t n = gn[j];
             if (a.raw != nullptr) {
                 a.raw[(uint64_t)c * N + n] = v;
-            } else if (c < 3) {
-                a.e_dyn[off + n] = a.e_stat[off + n] + a.ke * v;
-            } else {
-                a.b_dyn[off + n] = a.b_stat[off + n] + a.kmu * v;
+                continue;
             }
+            float* const dyn = c < 3 ? a.e_dyn : a.b_dyn;
+            if (a.judge != nullptr) {  // the surrogate's raw sum, about to be replaced
+                const float sur = dyn[off + n];
+                jd[c < 3 ? 0 : 2] = __builtin_fmaf(sur - v, sur - v, jd[c < 3 ? 0 : 2]);
+                jd[c < 3 ? 1 : 3] = __builtin_fmaf(v, v, jd[c < 3 ? 1 : 3]);
+            }
+            if (c < 3) dyn[off + n] = a.e_stat[off + n] + a.ke * v;
+            else dyn[off + n] = a.b_stat[off + n] + a.kmu * v;
         }
     }
+    if (a.judge != nullptr) judge_block_sums(jd, a.judge);
 }
 
 __global__ void k_combine_fields(FieldArgs a) {
@@ -514,14 +555,14 @@ hipError_t launch_prep_sources(const void* qu_lod, bool f64, uint32_t lo, uint32
     return hipGetLastError();
 }
 
-hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s) {
+// Launch shape of the field update: the LOD-tiled kernel with ROW cells per
+// work-item and TX x TY x TZ work-items per block (row 0: the generic kernel).
+struct FieldShape {
+    uint32_t row, TX, TY, TZ, blocks;
+};
+static FieldShape field_shape(const FieldArgs& a, bool fast) {
     constexpr int CPT_EXACT = 1, CPT_FAST = 2;
-    if (a.f64) {
-        hipLaunchKernelGGL(k_update_e_b_f64, dim3((uint32_t)((a.N + FIELD_BLOCK - 1) / FIELD_BLOCK)),
-                           dim3(FIELD_BLOCK), 0, s, a);
-        return hipGetLastError();
-    }
-    if (!a.near && a.D > 0) {
+    if (!a.f64 && !a.near && a.D > 0) {
         // LOD-aligned tiles: (ROW*TX) | lod block x, TY | lod block y, TZ | lod block z
         const uint32_t nd = 1u << a.D;
         const uint32_t lbx = a.nx / nd, lby = a.ny / nd, lbz = a.nz / nd;
@@ -530,21 +571,36 @@ hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s) {
             uint32_t TX, TY, TZ;
             tile_shape(lbx / row, lby, lbz, TX, TY, TZ);
             if (TX * TY * TZ < 64) continue;
-            const uint32_t blocks = (a.nx / (row * TX)) * (a.ny / TY) * (a.nz / TZ);
-            const dim3 g(blocks), bl(TX * TY * TZ);
-            if (fast) {
-                if (row == 8) hipLaunchKernelGGL((k_update_e_b_tiled<8, true>), g, bl, 0, s, a, TX, TY, TZ);
-                else hipLaunchKernelGGL((k_update_e_b_tiled<4, true>), g, bl, 0, s, a, TX, TY, TZ);
-            } else {
-                if (row == 8) hipLaunchKernelGGL((k_update_e_b_tiled<8, false>), g, bl, 0, s, a, TX, TY, TZ);
-                else hipLaunchKernelGGL((k_update_e_b_tiled<4, false>), g, bl, 0, s, a, TX, TY, TZ);
-            }
-            return hipGetLastError();
+            return FieldShape{row, TX, TY, TZ, (a.nx / (row * TX)) * (a.ny / TY) * (a.nz / TZ)};
         }
     }
-    const int cpt = fast ? CPT_FAST : CPT_EXACT;
-    const uint64_t per = (uint64_t)FIELD_BLOCK * cpt;
-    const dim3 g((uint32_t)((a.N + per - 1) / per)), b(FIELD_BLOCK);
+    const uint64_t per = (uint64_t)FIELD_BLOCK * (a.f64 ? 1 : (fast ? CPT_FAST : CPT_EXACT));
+    return FieldShape{0, 0, 0, 0, (uint32_t)((a.N + per - 1) / per)};
+}
+
+uint32_t field_blocks(const FieldArgs& a, bool fast) { return field_shape(a, fast).blocks; }
+
+hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s) {
+    constexpr int CPT_EXACT = 1, CPT_FAST = 2;
+    const FieldShape sh = field_shape(a, fast);
+    if (a.f64) {
+        if (a.judge != nullptr) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_update_e_b_f64, dim3(sh.blocks), dim3(FIELD_BLOCK), 0, s, a);
+        return hipGetLastError();
+    }
+    if (sh.row) {
+        const uint32_t TX = sh.TX, TY = sh.TY, TZ = sh.TZ;
+        const dim3 g(sh.blocks), bl(TX * TY * TZ);
+        if (fast) {
+            if (sh.row == 8) hipLaunchKernelGGL((k_update_e_b_tiled<8, true>), g, bl, 0, s, a, TX, TY, TZ);
+            else hipLaunchKernelGGL((k_update_e_b_tiled<4, true>), g, bl, 0, s, a, TX, TY, TZ);
+        } else {
+            if (sh.row == 8) hipLaunchKernelGGL((k_update_e_b_tiled<8, false>), g, bl, 0, s, a, TX, TY, TZ);
+            else hipLaunchKernelGGL((k_update_e_b_tiled<4, false>), g, bl, 0, s, a, TX, TY, TZ);
+        }
+        return hipGetLastError();
+    }
+    const dim3 g(sh.blocks), b(FIELD_BLOCK);
     if (fast) {
         if (a.near) hipLaunchKernelGGL((k_update_e_b<true, true, CPT_FAST>), g, b, 0, s, a);
         else hipLaunchKernelGGL((k_update_e_b<true, false, CPT_FAST>), g, b, 0, s, a);
@@ -552,6 +608,29 @@ hipError_t launch_update_e_b(const FieldArgs& a, bool fast, hipStream_t s) {
         if (a.near) hipLaunchKernelGGL((k_update_e_b<false, true, CPT_EXACT>), g, b, 0, s, a);
         else hipLaunchKernelGGL((k_update_e_b<false, false, CPT_EXACT>), g, b, 0, s, a);
     }
+    return hipGetLastError();
+}
+
+// Judge partials [nblocks][4] -> out[4]: thread t sums the blocks t, t + 256,
+// ... of component t % 4 group, then the 64 group sums of each component are
+// added in order (a fixed order: the same residual on every run).
+__global__ void __launch_bounds__(256) k_judge_reduce(const double* __restrict__ partial, uint32_t nblocks,
+                                                      double* __restrict__ out) {
+    __shared__ double sh[256];
+    const uint32_t t = threadIdx.x, comp = t & 3u, grp = t >> 2;
+    double acc = 0.0;
+    for (uint32_t b = grp; b < nblocks; b += 64u) acc += partial[(size_t)b * 4 + comp];
+    sh[t] = acc;
+    __syncthreads();
+    if (t < 4) {
+        double s = 0.0;
+        for (uint32_t g = 0; g < 64u; g++) s += sh[g * 4 + t];
+        out[t] = s;
+    }
+}
+
+hipError_t launch_judge_reduce(const double* partial, uint32_t nblocks, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_judge_reduce, dim3(1), dim3(256), 0, s, partial, nblocks, out);
     return hipGetLastError();
 }
 

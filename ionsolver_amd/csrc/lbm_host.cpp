@@ -8,6 +8,7 @@
 // format of src/file.rs / FILE_LAYOUT.txt. One handle = one HIP stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -103,6 +104,8 @@ struct ion_handle {
     // learned field surrogate and its judge
     ion::sur::Net* net = nullptr;
     float* field_raw = nullptr;  // raw direct source sums (E then B, 6 x N) once a surrogate exists
+    double* judge_partials = nullptr;  // the judge's residual sums [4] then [judge_blocks][4]
+    uint32_t judge_blocks = 0;
     int solver = ION_SOLVER_DIRECT;
     uint32_t judge_every = 50;
     double tolerance = 0.01;
@@ -253,9 +256,7 @@ int prep_sources(ion_handle* h) {
     });
 }
 
-int field_update(ion_handle* h) {
-    int rc = prep_sources(h);
-    if (rc) return rc;
+ion::FieldArgs field_args(ion_handle* h) {
     ion::FieldArgs f{};
     f.e_stat = h->e_stat;
     f.b_stat = h->b_stat;
@@ -284,6 +285,13 @@ int field_update(ion_handle* h) {
     f.kmu = h->kc.kmu;
     f.raw = h->field_raw;
     f.f64 = h->f64() ? 1u : 0u;
+    return f;
+}
+
+int field_update(ion_handle* h) {
+    int rc = prep_sources(h);
+    if (rc) return rc;
+    const ion::FieldArgs f = field_args(h);
     return timed(h, 3, [&] {
         hipError_t e = ion::launch_update_e_b(f, h->field_mode == ION_FIELD_FAST, h->stream);
         if (e == hipSuccess && f.raw != nullptr) e = ion::launch_combine_fields(f, h->stream);
@@ -291,9 +299,45 @@ int field_update(ion_handle* h) {
     });
 }
 
+
 ion::sur::Fields sur_fields(ion_handle* h) {
     return ion::sur::Fields{(const float4*)h->qu_lod, h->flags, h->e_stat, h->b_stat, h->e_dyn, h->b_dyn,
                             h->field_raw, h->src_pq, h->src_vd, h->src_count};
+}
+
+// The judge's field step: the surrogate writes its raw source sums into
+// E_dyn / B_dyn, then the direct kernel compares its own sums with them while
+// it overwrites them with the direct fields (bit-identical to a direct step),
+// so no 6 x N buffer is needed for either. Residuals: relative L2 of the
+// dynamic E and B over the fluid cells.
+int judged_field_update(ion_handle* h, double* rel_e, double* rel_b) {
+    int rc = prep_sources(h);
+    if (rc) return rc;
+    ion::FieldArgs f = field_args(h);
+    f.raw = nullptr;
+    const bool fast = h->field_mode == ION_FIELD_FAST;
+    if (!h->judge_partials) {
+        const uint32_t nb = std::max(ion::field_blocks(f, true), ion::field_blocks(f, false));
+        if ((rc = dalloc(h, (void**)&h->judge_partials, ((size_t)nb + 1) * 4 * sizeof(double)))) return rc;
+        h->judge_blocks = nb;
+    }
+    f.judge = h->judge_partials + 4;
+    bool ok = true;
+    rc = timed(h, 3, [&] {
+        ok = h->net->infer_raw(sur_fields(h)) == 0;
+        if (!ok) return hipSuccess;
+        hipError_t e = ion::launch_update_e_b(f, fast, h->stream);
+        if (e == hipSuccess) e = ion::launch_judge_reduce(f.judge, ion::field_blocks(f, fast), h->judge_partials, h->stream);
+        return e;
+    });
+    if (rc) return rc;
+    if (!ok) return fail(ION_ERR_DEVICE, "surrogate: " + h->net->error());
+    double s[4];
+    HIPCHK(hipMemcpyAsync(s, h->judge_partials, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *rel_e = s[1] > 0 ? std::sqrt(s[0] / s[1]) : 0.0;
+    *rel_b = s[3] > 0 ? std::sqrt(s[2] / s[3]) : 0.0;
+    return ION_OK;
 }
 
 // The field step of a time step: the direct kernel, or the surrogate with the
@@ -308,11 +352,10 @@ int field_step(ion_handle* h) {
     h->field_steps += 1;
     int rc;
     if (judge) {
-        if ((rc = field_update(h))) return rc;
+        double re = 0, rb = 0;
+        if ((rc = judged_field_update(h, &re, &rb))) return rc;
         h->sstats.direct_steps += 1;
         if (h->fallback_left > 0) h->fallback_left -= 1;
-        double re = 0, rb = 0;
-        if (h->net->residual(sur_fields(h), &re, &rb)) return fail(ION_ERR_DEVICE, h->net->error());
         h->sstats.last_residual_e = re;
         h->sstats.last_residual_b = rb;
         if (!(re <= h->tolerance && rb <= h->tolerance)) {  // a NaN residual is a miss too

@@ -48,14 +48,14 @@ __device__ __forceinline__ float far_at(const GemmOut& o, uint32_t m, uint32_t c
     lagrange3((float)ox, (float)o.bx, Lx);
     lagrange3((float)oy, (float)o.by, Ly);
     lagrange3((float)oz, (float)o.bz, Lz);
-    const float* f = o.far + (size_t)m * FAR_PTS * 6 + c;
+    const float* f = o.far + ((size_t)m * 6 + c) * FAR_PTS;
     float acc = 0.0f;
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++)
 #pragma unroll
-            for (int k = 0; k < 3; k++) acc = __builtin_fmaf(Lz[i] * Ly[j] * Lx[k], f[(i * 9 + j * 3 + k) * 6], acc);
+            for (int k = 0; k < 3; k++) acc = __builtin_fmaf(Lz[i] * Ly[j] * Lx[k], f[i * 9 + j * 3 + k], acc);
     return acc;
 }
 
@@ -88,7 +88,8 @@ __device__ __forceinline__ void epilogue(const GemmOut& out, int M, int N, int m
             const float* stat = c < 3 ? out.e_stat : out.b_stat;
             float* dyn = c < 3 ? out.e_dyn : out.b_dyn;
             const size_t idx = (size_t)(c % 3) * out.ncell + cell;
-            dyn[idx] = (stat[idx] + (*out.in_scale) * s * acc) + kf * far;
+            if (out.raw_out) dyn[idx] = (*out.in_scale) * acc / out.gain + far;
+            else dyn[idx] = (stat[idx] + (*out.in_scale) * s * acc) + kf * far;
         }
     }
 }
@@ -261,13 +262,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
             for (int e = lane; e < 16 * FAR_PTS; e += 64) {
                 const int t = e / FAR_PTS, p = e % FAR_PTS;
                 const int m = m0 + wm + i * 16 + t;
-                Fs[wave][t][p] = m < M ? out.far[((size_t)m * FAR_PTS + p) * 6 + hc] : 0.0f;
+                Fs[wave][t][p] = m < M ? out.far[((size_t)m * 6 + hc) * FAR_PTS + p] : 0.0f;
             }
         }
         __syncthreads();
         if constexpr (head_lds) {
             const int n = n0 + wn + lane;
             const bool train = out.fine != nullptr;
+            const bool raw_out = out.raw_out != 0;
+            const float rawk = in_s / out.gain;  // network output -> raw source sum
             const float sc = hc < 3 ? out.scale_e : out.scale_b;
             const float kf = hc < 3 ? out.ke : out.kmu;
             const float* stat = (hc < 3 ? out.e_stat : out.b_stat) + (size_t)(hc % 3) * out.ncell;
@@ -299,7 +302,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
                     for (int u = 0; u < 8; u++) {  // streamed once: nontemporal, so A stays in L2
                         fl[u] = __builtin_nontemporal_load(out.flags + cell[u]);
-                        st[u] = __builtin_nontemporal_load(stat + cell[u]);
+                        st[u] = raw_out ? 0.0f : __builtin_nontemporal_load(stat + cell[u]);
                     }
                 }
 #pragma unroll
@@ -322,7 +325,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
                     if (!valid[u]) continue;
                     if (train) __builtin_nontemporal_store(v + far * (out.gain / in_s), fine + cell[u]);
                     else if ((fl[u] & 0x1F) != 0x01)
-                        __builtin_nontemporal_store((st[u] + in_s * sc * v) + kf * far, dyn + cell[u]);
+                        __builtin_nontemporal_store(raw_out ? __builtin_fmaf(rawk, v, far) : (st[u] + in_s * sc * v) + kf * far,
+                                                    dyn + cell[u]);
                 }
             }
         } else {
@@ -410,7 +414,7 @@ __global__ void __launch_bounds__(FAR_THREADS) k_far_samples(FarArgs a) {
         float sum = 0.0f;
         for (uint32_t c = 0; c < FAR_CHUNKS; c++) sum += part[c][tid];
         const uint32_t r = tid / 18, k = (tid / 6) % 3, comp = tid % 6;
-        a.far[((size_t)blk * FAR_PTS + k + 3 * r) * 6 + comp] = sum;
+        a.far[((size_t)blk * 6 + comp) * FAR_PTS + k + 3 * r] = sum;
     }
 }
 

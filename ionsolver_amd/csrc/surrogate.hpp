@@ -27,7 +27,8 @@ struct GemmOut {
     const float* in_scale;    // input normalisation (device scalar)
     float scale_e, scale_b;   // KE / gain, KMU / gain (gain: network output units per raw field unit)
     float ke, kmu, gain;
-    const float* far;         // far-field samples [block][27][6], interpolated in the epilogue
+    const float* far;         // far-field samples [block][6][27], interpolated in the epilogue
+    int raw_out;              // write the raw source sums of the fluid cells into e_dyn / b_dyn (for the judge)
     uint64_t ncell;
     uint32_t nx, ny, nz, nd, bx, by, bz, cpb;
 };
@@ -46,7 +47,7 @@ struct FarArgs {
     const float4* src_pq;      // compacted LOD sources (position, charge)
     const float4* src_vd;      // (velocity, LOD index bits)
     const uint32_t* src_count;
-    float* far;                // [block][27][6]: E then B raw sums
+    float* far;                // [block][6][27]: E then B raw sums, the 27 points of a component together
     uint32_t nd, D, nblk;
     uint32_t lbx, lby, lbz;    // cells per block side
 };

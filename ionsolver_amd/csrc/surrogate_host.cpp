@@ -317,6 +317,15 @@ int Net::infer(const Fields& f) {
     return 0;
 }
 
+int Net::infer_raw(const Fields& f) {
+    int rc;
+    if ((rc = forward_features(f.qu)) || (rc = far_field(f))) return rc;
+    GemmOut o = head_out(f, nullptr);
+    o.raw_out = 1;
+    SCHK(gemm_nt(EPI_HEAD, hcol_, hw_, (int)nblk_, (int)nout_, hk_, hk_, hk_, o, stream_));
+    return 0;
+}
+
 int Net::loss(const Fields& f, double* rel_e, double* rel_b) {
     if (f.target == nullptr) {
         err_ = "no direct field to compare with";

@@ -48,6 +48,9 @@ class Net {
     int create(const Geom& geo, const Config& cfg, hipStream_t stream);
     // surrogate field: E_dyn / B_dyn of fluid cells from the LOD sums
     int infer(const Fields& f);
+    // The surrogate's raw source sums (the dynamic parts before the KE / KMU
+    // scaling) of the fluid cells, written into f.e_dyn / f.b_dyn.
+    int infer_raw(const Fields& f);
     // forward + relative L2 errors against the raw direct sums in f.target
     int residual(const Fields& f, double* rel_e, double* rel_b);
     // forward, loss against f.target, backward and (if update) one Adam step
@@ -93,7 +96,7 @@ class Net {
     float gain_ = 1.0f;  // network output = raw field sum * gain / s (gain = block side^2)
     Layer layers_[NLAYER];
     __bf16* x_ = nullptr;
-    float* far_ = nullptr;  // far-field samples [block][27][6]
+    float* far_ = nullptr;  // far-field samples [block][6][27]
     float* scale_ = nullptr;
     __bf16* pool_[3] = {nullptr, nullptr, nullptr};
     float* dpool_[3] = {nullptr, nullptr, nullptr};

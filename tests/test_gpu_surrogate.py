@@ -332,3 +332,36 @@ def test_trained_network_gains_over_the_operator_without_it():
         for lbm in (trainer, traj, held):
             if lbm is not None:
                 lbm.close()
+
+
+def test_judge_residual_matches_the_explicit_residual():
+    """The judge's residual is computed inside the direct kernel's epilogue
+    (the surrogate first writes its raw sums into E_dyn / B_dyn, the direct
+    kernel compares its own sums with them before overwriting them): it
+    agrees with ion_surrogate_residual (separate output buffer, network
+    units) on the same LOD sums, up to float rounding of the sums."""
+    cfg = make_config(nx=32, ny=32, nz=32, mhd=True, lod_depth=3, graphics=False)
+    lbm = I.Lbm(cfg)
+    try:
+        rng = np.random.default_rng(6)
+        n = 32 ** 3
+        lbm.write_field(Field.Q, (0.05 * rng.standard_normal(n)).astype(np.float32))
+        lbm.write_field(Field.U, (0.02 * rng.standard_normal(3 * n)).astype(np.float32))
+        lbm.write_field(Field.B_STAT, np.full(3 * n, 0.01, np.float32))
+        lbm.surrogate_create(channels=8)
+        lbm.set_field_solver(FieldSolver.SURROGATE, judge_every=1, tolerance=1.0)
+        lbm.initialize()
+        lbm.run(3)
+        st = lbm.surrogate_stats()
+        assert st["direct_steps"] >= 3 and st["fallbacks"] == 0
+        fused = np.array([st["last_residual_e"], st["last_residual_b"]])
+        e_dyn = lbm.read_field(Field.E_DYN)
+        lbm.write_field(Field.QU_LOD, lbm.read_field(Field.QU_LOD))
+        lbm.compute_fields()  # the same LOD sums: the same direct field
+        np.testing.assert_array_equal(lbm.read_field(Field.E_DYN), e_dyn)
+        explicit = np.array(lbm.surrogate_residual())
+        print("judge residual", fused, "explicit", explicit)
+        assert np.all(fused > 0) and np.all(fused < 1e-2)
+        np.testing.assert_allclose(fused, explicit, rtol=1e-3)
+    finally:
+        lbm.close()
