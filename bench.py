@@ -8,8 +8,8 @@ update_e_b_dynamic field step, over the whole lattice.
 Default workload (the configuration the headline metric is quoted on):
 256^3 D3Q19 SRT, FP32 DDFs, MHD with 8 seeded Gaussian charge blobs
 (sigma = 6 cells) on a neutral plasma background (ion charge density 1 +
-blobs, electron gas started at density 1, plasma units: KE 1.0e-3,
-KMU 2.8e-4, KKGE -2.5), uniform B0 plus a point dipole, Guo Lorentz forcing,
+blobs, electron gas started at density 1, plasma units: KE 1.0e-6,
+KMU 2.8e-5, KKGE -2.5), uniform B0 plus a point dipole, Guo Lorentz forcing,
 a solid cylinder r = 40, LOD depth 4. A workload name ending in "-residue"
 starts the electron gas at density 0 in the blob workloads' original units,
 as the reference's initialize does; that state turns NaN within about ten
@@ -374,6 +374,38 @@ def state_check(lbm, cfg):
     return out
 
 
+def stability(name, steps, every, field_mode):
+    """The workload's state every `every` steps over `steps` steps from
+    initialize (no timing): finiteness, peak |u| and |Q|, and the FP64 mass
+    and charge sums. How long a configuration stays physical."""
+    import ionsolver_amd as I
+    from ionsolver_amd import scenarios
+    cfg, arrays, meta = workload(name)
+    lbm = I.Lbm(cfg)
+    traj = []
+    try:
+        scenarios.apply(lbm, arrays)
+        del arrays
+        lbm.set_field_mode(field_mode)
+        lbm.initialize()
+        done = 0
+        while done < steps:
+            k = min(every, steps - done)
+            lbm.run(k)
+            done += k
+            st = state_check(lbm, cfg)
+            mass, charge = lbm.conservation()
+            traj.append({"step": done, **st, "mass": mass, "charge": charge})
+            print(f"stability step {done}/{steps}: {traj[-1]}", file=sys.stderr, flush=True)
+            if not st["finite"]:
+                break
+    finally:
+        lbm.close()
+    k = cfg.kernel_constants()
+    return {"workload": name, "lattice": [cfg.n_x, cfg.n_y, cfg.n_z], "steps": steps,
+            "units": {"ke": k["ke"], "kmu": k["kmu"], "kkge": k["kkge"]}, "meta": meta, "trajectory": traj}
+
+
 def fp64_error(name, steps, field_mode, n=128):
     """Relative L2 error of the measured configuration (same workload, storage
     format and field arithmetic, same number of steps) against the FP64
@@ -563,6 +595,8 @@ def main():
     ap.add_argument("--no-static-field", action="store_true",
                     help="skip the static-field precompute leg (setup_deeva_test precompute_B)")
     ap.add_argument("--static-field-only", action="store_true", help="print only the static-field leg")
+    ap.add_argument("--stability", type=int, default=0, metavar="EVERY",
+                    help="print only the state every EVERY steps over --steps steps (no timing)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run the HBM copy probe (3 copies of 1 GiB: reads 1 GiB, writes 1 GiB each): "
                          "the known-size kernel for PMC FETCH_SIZE/WRITE_SIZE scale factors")
@@ -582,6 +616,10 @@ def main():
         return
     if args.static_field_only:
         print(json.dumps(static_field_leg()), flush=True)
+        return
+    if args.stability:
+        mode = FieldMode.FAST if args.field_mode == "fast" else FieldMode.EXACT
+        print(json.dumps(stability(args.workload, args.steps, args.stability, mode)), flush=True)
         return
     if args.calibrate:
         I.hbm_copy_probe(1 << 30, 2)

@@ -17,13 +17,18 @@ from .lbm import FloatType, LbmConfig, ModelType, VelocitySet
 MHD_UNITS = dict(lbm_length=128.0, lbm_velocity=1.0, lbm_rho=1.0, lbm_charge=1.0, lbm_temp=1.0, si_length=0.1,
                  si_velocity=1.0, si_rho=1.2250, si_charge=7e-13, si_temp=1.0)
 
-# Units of the plasma workloads (Units fields set directly): KE = 1.0e-3,
-# KMU = 2.8e-4, KKGE = -2.50 in lattice units. With MHD_UNITS KKGE is
+# Units of the plasma workloads (Units fields set directly): KE = 1.0e-6,
+# KMU = 2.8e-5, KKGE = -2.50 in lattice units. With MHD_UNITS KKGE is
 # -6.8e-15, so the electron-gas velocity shift F / (2 rho_e KKGE)
 # (sim_kernels.cl:651-653) saturates at the clamp +-c for any field and the
 # electron gas runs away within a few steps whatever its density; here it
-# moves by (E + u_e x B) / 5 per step and stays in its stable range.
-PLASMA_UNITS = dict(m=1.58e8, kg=2.27e-12, s=1.0, a=1.0)
+# moves by (E + u_e x B) / 5 per step. The Coulomb constant sets how long the
+# state lives: with KE = 1.0e-3 the 256^3 blob workload's velocities reach
+# the clamp and its state turns NaN between steps 300 and 400, with
+# KE = 1.0e-5 its peak velocity still grows (0.9e-3 at step 500, 2.5e-3 at
+# step 3000), with KE = 1.0e-6 it stays below 2e-4 through 3000 steps while
+# the charge peaks relax (0.028 -> 0.004).
+PLASMA_UNITS = dict(m=1.58e9, kg=2.27e-12, s=1.0, a=1.0)
 
 
 def use_plasma_units(cfg, n0):
@@ -201,21 +206,29 @@ def _loop_field(r, z, a, wire):
     return br, bz
 
 
-def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magnet_r=350.0, b_center=0.05,
-             u_inlet=0.02, q_inlet=0.02, n_blobs=8, sigma=12.0, amp=0.05, seed=1234, log=None, plasma=None):
+def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magnet_r=350.0, b_center=None,
+             u_inlet=0.02, q_inlet=0.02, n_blobs=8, sigma=12.0, amp=0.05, seed=1234, log=None, plasma=None,
+             outlet=True):
     """Synthetic plasma-thruster workload: a z-aligned quartz tube (solid
     shell, flags 0x01) of radius tube_r, a ring magnet (current loop of
     radius magnet_r at mid-height, analytic field scaled to b_center at its
     centre) as B_stat, an inlet plane (flags 0x02, equilibrium boundary) at
-    z = 0 inside the tube with u = (0, 0, u_inlet), the tube filled with
-    fluid moving at u_inlet and seeded with Gaussian charge blobs. With
-    q_inlet set (None switches it off) the inlet also injects charge: the
-    charge inlet extension holds the inlet cells at charge density q_inlet.
-    With plasma = n0 the charges sit on a neutral background (as in
-    mhd_blobs): PLASMA_UNITS, the electron gas of every non-solid cell starts
-    at density n0 and the ion charge density written to Q is n0 plus the
-    above. Lengths are given for n = 1024 and scaled with n. Arrays are built
-    one z-layer at a time. Returns (cfg, arrays, meta)."""
+    z = 0 inside the tube with u = (0, 0, u_inlet), with outlet an outlet
+    plane of the same kind at z = n - 1, the tube filled with fluid moving
+    at u_inlet and seeded with Gaussian charge blobs. With q_inlet set (None
+    switches it off) the inlet also injects charge: the charge inlet
+    extension holds the inlet cells at charge density q_inlet and the outlet
+    cells at the background density. With plasma = n0 the charges sit on a
+    neutral background (as in mhd_blobs): PLASMA_UNITS, the electron gas of
+    every non-solid cell starts at density n0 and the ion charge density
+    written to Q is n0 plus the above. b_center defaults to 0.05 without the
+    plasma background and to 0.002 with it: the gas then carries the
+    background's charge density n0 = 1, and at 0.05 the Lorentz force
+    n0 (u x B) drives the axial velocity at the magnet to the clamp within
+    about 300 steps at 256^3 (ion gyration frequency n0 B / rho = 0.05 per
+    step), while at 0.002 the flow develops as in the uncharged tube. Lengths
+    are given for n = 1024 and scaled with n. Arrays are built one z-layer at
+    a time. Returns (cfg, arrays, meta)."""
     from .lbm import FloatType as FT
     ft = FT.FP16C if float_type is None else float_type
     sc = n / 1024.0
@@ -234,6 +247,8 @@ def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magne
     if plasma is not None:
         use_plasma_units(cfg, plasma)
     n0 = 0.0 if plasma is None else plasma
+    if b_center is None:
+        b_center = 0.05 if plasma is None else 0.002
     N = n ** 3
     c = 0.5 * n
     yy, xx = np.meshgrid(np.arange(n, dtype=np.float64) - c, np.arange(n, dtype=np.float64) - c, indexing="ij")
@@ -269,10 +284,12 @@ def thruster(n=1024, float_type=None, lod_depth=4, tube_r=300.0, wall=4.0, magne
     q_in = (np.where(inside, q_inlet, 0.0).ravel() + q_back) if q_inlet is not None else None
     for z in range(n):
         sl = slice(z * lay, (z + 1) * lay)
-        flags[sl] = f_inlet if z == 0 else f_layer
+        flags[sl] = f_inlet if z == 0 or (outlet and z == n - 1) else f_layer
         u[2 * N + z * lay:2 * N + (z + 1) * lay] = uz_layer
         if z == 0 and q_in is not None:
             q[sl] = q_in
+        elif z == n - 1 and outlet:
+            q[sl] = q_back
         else:
             wz = np.array([np.exp(-((z - bz) ** 2) / s2) for bx, by, bz, a in blobs])
             q[sl] = wz @ gxy + q_back

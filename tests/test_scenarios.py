@@ -35,7 +35,8 @@ def test_thruster_geometry():
     x = np.arange(n) - n / 2
     r = np.sqrt(x[None, :] ** 2 + x[:, None] ** 2)
     assert np.all(f[:, (r >= meta["tube_r"]) & (r < meta["tube_r"] + meta["wall"])] == 1)
-    assert np.all(f[0][r < meta["tube_r"]] == 2) and np.all(f[1:][:, r < meta["tube_r"]] == 0)
+    inside = r < meta["tube_r"]
+    assert np.all(f[0][inside] == 2) and np.all(f[-1][inside] == 2) and np.all(f[1:-1][:, inside] == 0)
     q = arr["q"].reshape(n, n, n)
     assert np.all(q[:, r >= meta["tube_r"]] == 0) and np.abs(q).max() > 0
     b = arr["b_stat"].reshape(3, n, n, n)
@@ -44,7 +45,8 @@ def test_thruster_geometry():
     np.testing.assert_allclose(b[2, n // 2 + 5], b[2, n // 2 - 5], rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(b[0, n // 2 + 5], -b[0, n // 2 - 5], rtol=1e-5, atol=1e-9)
     assert cfg.ext_equilibrium_boudaries and cfg.ext_magneto_hydro and cfg.ext_charge_inlet
-    assert np.all(q[0][r < meta["tube_r"]] == np.float32(meta["q_inlet"]))
+    assert np.all(q[0][inside] == np.float32(meta["q_inlet"])) and np.all(q[-1][inside] == 0)
+    assert meta["b_center"] == 0.05
 
 
 def test_subgrid_ecr_workload_setup():
@@ -71,3 +73,20 @@ def test_subgrid_ecr_workload_setup():
     s = Sink()
     apply(s, arrays)
     assert s.seen[Field.E_VAR] is arrays["e_var"] and s.seen[Field.ET] is arrays["et"]
+
+
+def test_plasma_thruster_defaults():
+    """With the plasma background the ring magnet is weaker (0.002 at its
+    centre: the gas carries the background charge density) and the outlet
+    plane holds the background charge density."""
+    n = 32
+    cfg, arr, meta = thruster(n=n, lod_depth=2, plasma=1.0)
+    assert meta["b_center"] == 0.002 and cfg.electron_density0 == 1.0
+    f = arr["flags"].reshape(n, n, n)
+    q = arr["q"].reshape(n, n, n)
+    x = np.arange(n) - n / 2
+    inside = np.sqrt(x[None, :] ** 2 + x[:, None] ** 2) < meta["tube_r"]
+    assert np.all(f[-1][inside] == 2) and np.all(q[-1][inside] == 1.0)
+    assert np.all(q[0][inside] == np.float32(1.0 + meta["q_inlet"]))
+    b = arr["b_stat"].reshape(3, n, n, n)
+    np.testing.assert_allclose(b[2, n // 2, n // 2, n // 2], 0.002, rtol=1e-5)
