@@ -79,7 +79,7 @@ __device__ __forceinline__ void epilogue(const GemmOut& out, int M, int N, int m
             (uint64_t)(X * out.bx + ox) + ((uint64_t)(Y * out.by + oy) + (uint64_t)(Z * out.bz + oz) * out.ny) * out.nx;
         if (out.fine != nullptr) {  // training: network-unit output (net + far field)
             const float far = far_at(out, (uint32_t)m, c, ox, oy, oz);
-            out.fine[(size_t)c * out.ncell + cell] = acc + far * (out.gain / *out.in_scale);
+            out.fine[(size_t)c * out.ncell + cell] = acc + far * (out.gain / out.in_scale[c < 3 ? 0 : 1]);
         } else {
             if ((out.flags[cell] & 0x1F) == 0x01) return;
             const float far = far_at(out, (uint32_t)m, c, ox, oy, oz);
@@ -88,8 +88,9 @@ __device__ __forceinline__ void epilogue(const GemmOut& out, int M, int N, int m
             const float* stat = c < 3 ? out.e_stat : out.b_stat;
             float* dyn = c < 3 ? out.e_dyn : out.b_dyn;
             const size_t idx = (size_t)(c % 3) * out.ncell + cell;
-            if (out.raw_out) dyn[idx] = (*out.in_scale) * acc / out.gain + far;
-            else dyn[idx] = (stat[idx] + (*out.in_scale) * s * acc) + kf * far;
+            const float in_s = out.in_scale[c < 3 ? 0 : 1];
+            if (out.raw_out) dyn[idx] = in_s * acc / out.gain + far;
+            else dyn[idx] = (stat[idx] + in_s * s * acc) + kf * far;
         }
     }
 }
@@ -241,9 +242,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
     uint32_t hc = 0, hcell0 = 0;
     float in_s = 1.0f;
     if (head_lds) {
-        in_s = *out.in_scale;
         const uint32_t n = (uint32_t)(n0 + wn + lane);
         hc = (uint32_t)(n0 + wn) / out.cpb;
+        in_s = out.in_scale[hc < 3 ? 0 : 1];
         const uint32_t o = n - hc * out.cpb;
         const uint32_t ox = o % out.bx, oy = (o / out.bx) % out.by, oz = o / (out.bx * out.by);
         lagrange3((float)ox, (float)out.bx, Lx);
@@ -424,25 +425,40 @@ __global__ void __launch_bounds__(FAR_THREADS) k_far_samples(FarArgs a) {
 // field update never reads (sim_kernels.cl:943).
 // ---------------------------------------------------------------------------
 __global__ void k_input_scale(const float4* __restrict__ qu, uint32_t n_top, float* __restrict__ scale) {
-    __shared__ float sh[256];
-    float m = 0.0f;
-    for (uint32_t i = threadIdx.x; i < n_top; i += blockDim.x) m = fmaxf(m, fabsf(qu[i].x));
-    sh[threadIdx.x] = m;
+    // scale[0] = max |q|, scale[1] = max |J| (J = q v, the float32 products
+    // k_input forms); a zero maximum scales by 1
+    __shared__ float sh[2][256];
+    float mq = 0.0f, mj = 0.0f;
+    for (uint32_t i = threadIdx.x; i < n_top; i += blockDim.x) {
+        const float4 e = qu[i];
+        mq = fmaxf(mq, fabsf(e.x));
+        mj = fmaxf(mj, fmaxf(fabsf(e.x * e.y), fmaxf(fabsf(e.x * e.z), fabsf(e.x * e.w))));
+    }
+    sh[0][threadIdx.x] = mq;
+    sh[1][threadIdx.x] = mj;
     __syncthreads();
     for (uint32_t s = 128; s >= 1; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
+        if (threadIdx.x < s) {
+            sh[0][threadIdx.x] = fmaxf(sh[0][threadIdx.x], sh[0][threadIdx.x + s]);
+            sh[1][threadIdx.x] = fmaxf(sh[1][threadIdx.x], sh[1][threadIdx.x + s]);
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *scale = sh[0] > 0.0f ? sh[0] : 1.0f;
+    if (threadIdx.x == 0) {
+        scale[0] = sh[0][0] > 0.0f ? sh[0][0] : 1.0f;
+        scale[1] = sh[1][0] > 0.0f ? sh[1][0] : 1.0f;
+    }
 }
 
+// The charge channel over scale[0], the current channels over scale[1]: E is
+// linear in q and B in J, so each output group is scale-free on its own.
 __global__ void k_input(const float4* __restrict__ qu, uint32_t n_top, uint32_t lo, const float* __restrict__ scale,
                         __bf16* __restrict__ x) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n_top) return;
     const float4 e = qu[v];
-    const float inv = 1.0f / *scale;
-    float in[4] = {e.x * inv, e.x * e.y * inv, e.x * e.z * inv, e.x * e.w * inv};
+    const float inv_q = 1.0f / scale[0], inv_j = 1.0f / scale[1];
+    float in[4] = {e.x * inv_q, e.x * e.y * inv_j, e.x * e.z * inv_j, e.x * e.w * inv_j};
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         const float val = v < lo ? 0.0f : in[c];
@@ -563,7 +579,8 @@ __global__ void k_to_bf16(const float* __restrict__ in, int rows, int cols, int 
 
 // Loss of the head output y (network units, fp32 [6][N]) against the direct
 // field's raw source sums (E then B, [6][N]) in network units,
-// t = raw * gain / s, over fluid cells: per-block partial sums of (y - t)^2
+// t = raw * gain / s (s = the charge scale for E, the current scale for B),
+// over fluid cells: per-block partial sums of (y - t)^2
 // and t^2 for the E and B groups.
 __global__ void __launch_bounds__(256) k_loss_sums(const float* __restrict__ y, const float* __restrict__ target,
                                                    const uint8_t* __restrict__ flags, uint64_t ncell,
@@ -571,12 +588,12 @@ __global__ void __launch_bounds__(256) k_loss_sums(const float* __restrict__ y, 
                                                    double* __restrict__ partial) {
     __shared__ double sh[4][256];
     double s[4] = {0, 0, 0, 0};
-    const float tg = gain / *in_scale;
+    const float tge = gain / in_scale[0], tgb = gain / in_scale[1];
     for (uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; n < ncell; n += (uint64_t)gridDim.x * blockDim.x) {
         if ((flags[n] & 0x1F) == 0x01) continue;
         for (int c = 0; c < 6; c++) {
             const size_t i = (size_t)c * ncell + n;
-            const float t = target[i] * tg;
+            const float t = target[i] * (c < 3 ? tge : tgb);
             const float d = y[i] - t;
             s[(c < 3 ? 0 : 2) + 0] += (double)d * d;
             s[(c < 3 ? 0 : 2) + 1] += (double)t * t;
@@ -601,7 +618,7 @@ __global__ void k_loss_grad(const float* __restrict__ y, const float* __restrict
                             float gain, const float* __restrict__ norms, __bf16* __restrict__ dY,
                             __bf16* __restrict__ dYt) {
     const uint64_t ncell = hg.ncell;
-    const float tg = gain / *in_scale;
+    const float tge = gain / in_scale[0], tgb = gain / in_scale[1];
     const float ge = norms[0] > 0.0f ? 2.0f / norms[0] : 0.0f, gb = norms[1] > 0.0f ? 2.0f / norms[1] : 0.0f;
     for (uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; n < ncell; n += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t z = (uint32_t)(n / ((uint64_t)hg.nx * hg.ny)), r = (uint32_t)(n % ((uint64_t)hg.nx * hg.ny));
@@ -611,7 +628,7 @@ __global__ void k_loss_grad(const float* __restrict__ y, const float* __restrict
         const bool fluid = (flags[n] & 0x1F) != 0x01;
         for (int c = 0; c < 6; c++) {
             const size_t i = (size_t)c * ncell + n;
-            const float g = fluid ? (c < 3 ? ge : gb) * (y[i] - target[i] * tg) : 0.0f;
+            const float g = fluid ? (c < 3 ? ge : gb) * (y[i] - target[i] * (c < 3 ? tge : tgb)) : 0.0f;
             const uint32_t col = c * hg.cpb + o;
             dY[(size_t)blk * hg.ld_dy + col] = f2bf(g);
             dYt[(size_t)col * hg.ld_dyt + blk] = f2bf(g);

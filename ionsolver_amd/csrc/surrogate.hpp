@@ -24,7 +24,7 @@ struct GemmOut {
     const float* e_stat;
     const float* b_stat;
     const uint8_t* flags;
-    const float* in_scale;    // input normalisation (device scalar)
+    const float* in_scale;    // input normalisation (device [2]: charge scale for E, current scale for B)
     float scale_e, scale_b;   // KE / gain, KMU / gain (gain: network output units per raw field unit)
     float ke, kmu, gain;
     const float* far;         // far-field samples [block][6][27], interpolated in the epilogue

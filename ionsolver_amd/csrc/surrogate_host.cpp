@@ -5,7 +5,7 @@
 //
 // Network (all convolutions 3^3, zero padding, no bias, no activation; the
 // operator it approximates is linear in the LOD charges and currents):
-//   x   = (q, Jx, Jy, Jz) of the top-level LODs / s, as bf16 hi+lo (8 ch)  grid g0 = nd^3
+//   x   = (q / s_q, J / s_J) of the top-level LODs, as bf16 hi+lo (8 ch)  grid g0 = nd^3
 //   e0  = conv(x)            C0   g0        d2 = conv([up(e3), e2])  C2   g2
 //   e1  = conv(pool(e0))     C1   g1        d1 = conv([up(d2), e1])  C1   g1
 //   e2  = conv(pool(e1))     C2   g2        d0 = conv([up(d1), e0])  C0   g0

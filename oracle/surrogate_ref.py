@@ -2,7 +2,8 @@
 the checker of the GPU network's forward output and gradients).
 
 Restates the network of ionsolver_amd/csrc/surrogate_host.cpp in numpy:
-input (q, J = q v) of the top-level LOD blocks / s as bf16 hi + lo parts,
+input (q / s_q, J / s_J, J = q v) of the top-level LOD blocks as bf16 hi +
+lo parts (s_q = max |q|, s_J = max |J| over blocks and components),
 a linear 4-level 3D conv U-Net over the LOD grid (3^3 convolutions, 2^3
 average pooling, nearest upsampling, skip concatenation), and the sub-pixel
 output head that maps the 3^3 neighbourhood of [d0, x] to the 6 field
@@ -116,18 +117,26 @@ def up_bwd(d):
 
 
 def net_input(geo, qu):
-    """(nblk, 4) LOD sums -> (g, g, g, 8) input, scale s."""
-    q = qu[:geo.nblk].astype(np.float64)
-    s = np.max(np.abs(np.float32(q[:, 0])))
-    s = float(s) if s > 0 else 1.0
-    v = np.stack([q[:, 0], q[:, 0] * q[:, 1], q[:, 0] * q[:, 2], q[:, 0] * q[:, 3]], 1)
-    v = np.float32(np.float32(v) * np.float32(1.0 / np.float32(s))).astype(np.float64)
+    """(nblk, 4) LOD sums -> (g, g, g, 8) input, scales (s_q, s_J): the charge
+    channel over s_q = max |q|, the three current channels J = q v (float32
+    products) over s_J = max |J|; a zero maximum scales by 1. E is linear in
+    q and B in J, so each group of outputs is scale-free on its own however
+    small J is next to q."""
+    q = qu[:geo.nblk].astype(np.float32)
+    J = q[:, 0:1] * q[:, 1:4]                                   # float32 products, as the GPU forms them
+    s_q = float(np.max(np.abs(q[:, 0])))
+    s_J = float(np.max(np.abs(J)))
+    s_q = s_q if s_q > 0 else 1.0
+    s_J = s_J if s_J > 0 else 1.0
+    inv = np.float32([1.0 / np.float32(s_q)] + [1.0 / np.float32(s_J)] * 3)
+    v = np.concatenate([q[:, 0:1], J], 1)
+    v = np.float32(v * inv).astype(np.float64)
     v[:geo.lo] = 0.0
     hi = bf16(v)
     lo = bf16(v - hi)
     x = np.concatenate([hi, lo], 1)
     g = geo.nd
-    return x.reshape(g, g, g, 8), s
+    return x.reshape(g, g, g, 8), (s_q, s_J)
 
 
 def forward(geo, params, qu):
@@ -171,8 +180,17 @@ def cells_to_block(geo, F):
 
 def targets(geo, s, raw):
     """Network-unit targets from the direct kernel's raw source sums
-    (E then B, 6 x N): raw * gain / s, gain = (cells per block)^(2/3)."""
-    return raw.reshape(6, -1).astype(np.float64) * (geo.gain / s)
+    (E then B, 6 x N): E rows raw * gain / s_q, B rows raw * gain / s_J,
+    gain = (cells per block)^(2/3)."""
+    t = raw.reshape(6, -1).astype(np.float64).copy()
+    t[:3] *= geo.gain / s[0]
+    t[3:] *= geo.gain / s[1]
+    return t
+
+
+def net_units(geo, s, f):
+    """(6, N) raw-unit field -> network units (the scaling of targets)."""
+    return targets(geo, s, f)
 
 
 def far_samples(geo, qu):

@@ -95,12 +95,12 @@ def test_forward_loss_and_gradients_vs_float64_reference(n, solid_frac):
 
     k = lbm.config.kernel_constants()
     fluid = (lbm.read_field(Field.FLAGS) & 0x1F) != 1
-    s = float(np.max(np.abs(top[:, 0])))
+    s = S.net_input(geo, top)[1]  # (charge scale, current scale)
     T = S.targets(geo, s, raw)
     assert np.sum(T[3:] ** 2) > 0
     far = S.far_interp(geo, S.far_samples(geo, top))
     L, g_ref, rel_ref, y_ref = S.loss_and_grad(geo, params.astype(np.float64), top, T, fluid.astype(np.float64),
-                                               y_add=far * (geo.gain / s))
+                                               y_add=S.net_units(geo, s, far))
     assert rel_e == pytest.approx(rel_ref[0], rel=2e-2)
     assert rel_b == pytest.approx(rel_ref[1], rel=2e-2)
     offs = np.cumsum([0] + [r * c for r, c in geo.shapes])
@@ -110,10 +110,10 @@ def test_forward_loss_and_gradients_vs_float64_reference(n, solid_frac):
     # the parameters were not touched by a step without update
     np.testing.assert_array_equal(lbm.surrogate_params(), params)
 
-    # inference writes E_stat + KE (s y_net / gain + far) = E_stat + KE s y / gain (fluid cells only)
+    # inference writes E_stat + KE (s_q y_net / gain + far) = E_stat + KE s_q y / gain (fluid cells only)
     lbm.surrogate_infer()
     e_sur = lbm.read_field(Field.E_DYN)
-    ref = e_st.astype(np.float64) + k["ke"] * s / geo.gain * y_ref[:3].ravel()
+    ref = e_st.astype(np.float64) + k["ke"] * s[0] / geo.gain * y_ref[:3].ravel()
     m3 = np.tile(fluid, 3)
     assert rel_l2((e_sur - e_st)[m3], (ref - e_st)[m3]) < 2e-2
     np.testing.assert_array_equal(e_sur[~m3], e_dyn[~m3])

@@ -84,14 +84,40 @@ def test_net_input_scaling_and_lo_mask():
     geo = S.Geometry(16, 16, 16, 3, 8)
     rng = np.random.default_rng(3)
     qu = rng.standard_normal((geo.nblk, 4)).astype(np.float32)
+    qu[:, 1:] *= 1e-4                                   # currents far below the charges
     x, s = S.net_input(geo, qu)
-    assert s == np.max(np.abs(qu[:, 0]))
+    J = np.stack([qu[:, 0] * qu[:, 1], qu[:, 0] * qu[:, 2], qu[:, 0] * qu[:, 3]], 1).astype(np.float64)
+    assert s[0] == np.max(np.abs(qu[:, 0])) and s[1] == pytest.approx(np.max(np.abs(J)), rel=1e-6)
     flat = x.reshape(-1, 8)
     assert np.all(flat[:geo.lo] == 0.0) and np.any(flat[geo.lo:] != 0.0)
     v = flat[:, :4] + flat[:, 4:]                       # hi + lo parts
-    ref = np.stack([qu[:, 0], qu[:, 0] * qu[:, 1], qu[:, 0] * qu[:, 2], qu[:, 0] * qu[:, 3]], 1) / s
+    ref = np.concatenate([qu[:, :1] / s[0], J / s[1]], 1)
     np.testing.assert_allclose(v[geo.lo:], ref[geo.lo:], rtol=2e-5, atol=1e-7)
+    # each group spans [-1, 1] whatever the ratio of the currents to the charges
+    assert np.max(np.abs(v[:, 0])) == pytest.approx(1.0, rel=1e-4)
+    assert np.max(np.abs(v[:, 1:])) == pytest.approx(1.0, rel=1e-4)
     assert geo.lo == 1 + 8 + 64
+    # no current at all: the current scale falls back to 1
+    qu[:, 1:] = 0.0
+    assert S.net_input(geo, qu)[1][1] == 1.0
+
+
+def test_output_groups_scale_separately():
+    """Scaling the currents by a factor scales the B output by it and leaves
+    E unchanged, scaling the charges at fixed velocity scales both (E in q,
+    B in J = q v): the two input scales keep each group scale-free."""
+    geo, params, _, _, _ = _problem(21)
+    rng = np.random.default_rng(22)
+    qu = np.zeros((geo.nblk, 4), np.float32)
+    qu[geo.lo:, 0] = np.float32(rng.choice([-1.0, 0.5, 1.0, 2.0], geo.nblk - geo.lo))
+    qu[geo.lo:, 1:] = np.float32(rng.choice([-0.25, 0.5, 1.0], (geo.nblk - geo.lo, 3)))
+    y0, s0, _ = S.forward(geo, params, qu)
+    qv = qu.copy()
+    qv[:, 1:] *= np.float32(2.0 ** -14)                # slower by an exact power of two
+    y1, s1, _ = S.forward(geo, params, qv)
+    assert s1[0] == s0[0] and s1[1] == s0[1] * 2.0 ** -14
+    # network units are the same; raw units follow the scales
+    np.testing.assert_allclose(y1, y0, rtol=1e-12, atol=1e-12)
 
 
 def _problem(seed, c0=8):
