@@ -40,12 +40,10 @@ FLOP_PER_INTERACTION = 35  # r 3, r^2 5, sqrt 1, cube 2, r/|r|^3 3, e 6, v x pre
 # by the factors the same counters give for ion_hbm_copy_probe's 1 GiB copy
 # (FETCH_SIZE x 2, WRITE_SIZE x 1). Kernels compiled since may differ.
 PMC_TRAFFIC = {
-    ("mhd256", "stream_collide"): 6.632e9,
-    ("mhd256-fp16c", "stream_collide"): 3.952e9,
-    ("tgv256", "stream_collide"): 2.910e9,
-    ("mhd256", "update_e_b_dynamic"): 1.292e9,
-    ("mhd256-fp16c", "update_e_b_dynamic"): 1.292e9,
-    ("mhd256-fp16s", "update_e_b_dynamic"): 1.292e9,
+    ("mhd256", "stream_collide"): 6.493e9,
+    ("mhd256-fp16c", "stream_collide"): 3.815e9,
+    ("mhd256", "update_e_b_dynamic"): 1.438e9,
+    ("mhd256-fp16c", "update_e_b_dynamic"): 1.438e9,
 }
 
 
@@ -575,6 +573,7 @@ def fp16c_leg(name, steps, warmup, field_mode):
     finally:
         lbm.close()
     sc = m["sc"]
+    sc["traffic"] = PMC_TRAFFIC.get((name, "stream_collide"))
     pull = pull_probe_gbs(2, 45, m["cfg"])
     sc["frac_of_pull_probe"] = round(sc["GB/s"] / pull, 4) if sc["GB/s"] else None
     return {"workload": name, "value": round(m["mlups"], 3), "unit": "MLUPs/s", "steps": steps, "warmup": warmup,
