@@ -266,11 +266,11 @@ def test_trained_network_gains_over_the_operator_without_it():
     held-out residual (a random configuration set and a trajectory of another
     blob seed) is below that of
     the same operator with the U-Net's contribution zeroed: by at least 20 %
-    for E (measured: 46 %). B's far field is smoother and its interpolation
-    leaves little for the network to learn: its residual stays within 15 % of
-    the operator's without the network (measured: 9 % above it, 0.17 %
-    against 0.16 %, far inside the judge's 1 %). Trajectories are sampled
-    after 20 steps, once the fields have set the gas moving."""
+    for E and for B (measured: 46 % and 44 %; E 0.052 % against 0.095 %, B
+    0.078 % against 0.138 %). The inputs are scaled per group (charges and
+    currents), so B's residual is measured on its own scale although the
+    plasma's currents are 1e-4 of its charges. Trajectories are sampled after
+    20 steps, once the fields have set the gas moving."""
     from ionsolver_amd import FloatType
     from ionsolver_amd.scenarios import apply, mhd_blobs
     n, D, c0 = 128, 4, 16
@@ -327,7 +327,7 @@ def test_trained_network_gains_over_the_operator_without_it():
         print("held-out residuals (E, B): trained", trained.mean(0), "without the U-Net", without.mean(0))
         assert np.all(trained < 1e-2)
         assert trained[:, 0].mean() < 0.8 * without[:, 0].mean()
-        assert trained[:, 1].mean() < 1.15 * without[:, 1].mean()
+        assert trained[:, 1].mean() < 0.8 * without[:, 1].mean()
     finally:
         for lbm in (trainer, traj, held):
             if lbm is not None:
