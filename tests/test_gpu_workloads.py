@@ -10,11 +10,15 @@
   the NaN it produces spreads (NaN positions must match too);
 * the NaN/Inf watchdog stops that run at its first non-finite charge sum;
 * storage formats on the 128^3 plasma blob workload (LOD depth 4, EXACT
-  field arithmetic, 50 steps): the relative L2 error of rho - 1, u, E - E_stat,
-  B - B_stat and Q of FP32, FP16S and FP16C storage against the FP64
-  instantiation of the same kernels, under the bounds stated below (B - B_stat
-  is about 1e-5 of B_stat there, so the f32 rounding of the stored B_dyn,
-  about 2.3e-3 of it, dominates its error in every format);
+  field arithmetic, 50 steps, strongly coupled plasma units): the relative L2
+  error of rho - 1, u, E - E_stat, B - B_stat and Q of FP32, FP16S and FP16C
+  storage against the FP64 instantiation of the same kernels, under the
+  bounds stated below (B - B_stat is about 1e-5 of B_stat there, so the f32
+  rounding of the stored B_dyn, about 2.3e-3 of it, dominates its error in
+  every format);
+* the plasma workloads in their long-run units stay physical over 1500
+  steps at 128^3: the blob workload's flow stays slow and its charge peaks
+  relax, the thruster's flow stays far below the velocity clamp;
 * the blob workload at 384^3 (LOD depth 4): FAST field arithmetic within
   1e-5 of EXACT on identical LOD sums after 4 plasma steps (initialize's
   field update sees no LOD sums, so the first field that moves the gas is
@@ -264,3 +268,46 @@ def test_runs_are_bitwise_reproducible():
     for x, y in zip(a, b):
         assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
     assert sa == sb and sa["surrogate_steps"] >= 4 and sa["direct_steps"] >= 4
+
+
+def test_plasma_workloads_stay_physical_for_1500_steps():
+    """PLASMA_UNITS (KE = 1e-6) and the plasma thruster's 0.002 ring magnet
+    were chosen so that long runs stay physical (with KE = 1e-3 the 256^3
+    blob workload is NaN by step 400, and with a 0.05 magnet the thruster's
+    axial velocity reaches the clamp within about 300 steps). 128^3, LOD
+    depth 4, direct field steps: the blob workload's peak |u| stays below
+    1e-3 and its peak |Q| does not grow; the thruster's peak |u| stays below
+    0.1 (inlet 0.02, clamp 1/sqrt(3)) with its state finite."""
+    n = 128
+    cfg, arrays, _ = scenarios.mhd_blobs(n=n, lod_depth=4, plasma=1.0)
+    lbm = I.Lbm(cfg)
+    try:
+        scenarios.apply(lbm, arrays)
+        lbm.set_field_mode(FieldMode.FAST)
+        lbm.initialize()
+        lbm.run(20)
+        q0 = np.abs(lbm.read_field(Field.Q)).max()
+        lbm.run(1480)
+        u = lbm.read_field(Field.U)
+        q = lbm.read_field(Field.Q)
+        assert np.isfinite(u).all() and np.isfinite(q).all()
+        print("blobs 128^3 after 1500 steps: |u|max", np.abs(u).max(), "|Q|max", np.abs(q).max(), "from", q0)
+        assert np.abs(u).max() < 1e-3
+        assert np.abs(q).max() <= q0
+    finally:
+        lbm.close()
+    cfg, arrays, meta = scenarios.thruster(n=n, plasma=1.0)
+    assert meta["b_center"] == 0.002
+    lbm = I.Lbm(cfg)
+    try:
+        scenarios.apply(lbm, arrays)
+        lbm.set_field_mode(FieldMode.FAST)
+        lbm.initialize()
+        lbm.run(1500)
+        u = lbm.read_field(Field.U)
+        q = lbm.read_field(Field.Q)
+        assert np.isfinite(u).all() and np.isfinite(q).all()
+        print("thruster 128^3 after 1500 steps: |u|max", np.abs(u).max())
+        assert np.abs(u).max() < 0.1
+    finally:
+        lbm.close()

@@ -1,5 +1,6 @@
 """Host-side workload generators (CPU only)."""
 import numpy as np
+import pytest
 
 from ionsolver_amd import FloatType
 from ionsolver_amd.scenarios import _loop_field, thruster
@@ -90,3 +91,18 @@ def test_plasma_thruster_defaults():
     assert np.all(q[0][inside] == np.float32(1.0 + meta["q_inlet"]))
     b = arr["b_stat"].reshape(3, n, n, n)
     np.testing.assert_allclose(b[2, n // 2, n // 2, n // 2], 0.002, rtol=1e-5)
+
+
+def test_plasma_unit_constants():
+    """The plasma workloads' lattice constants: KE = 1.0e-6, KMU = 2.8e-5,
+    KKGE = -2.50 (long-run units) and KE = 1.0e-3, KMU = 2.8e-4 (strongly
+    coupled units, for short accuracy comparisons)."""
+    from ionsolver_amd import scenarios
+    cfg, _, _ = scenarios.mhd_blobs(n=64, lod_depth=2, plasma=1.0)
+    k = cfg.kernel_constants()
+    assert k["ke"] == pytest.approx(1.0e-6, rel=0.01) and k["kmu"] == pytest.approx(2.8e-5, rel=0.01)
+    assert k["kkge"] == pytest.approx(-2.50, rel=0.01) and cfg.electron_density0 == 1.0
+    scenarios.use_plasma_units(cfg, 1.0, scenarios.PLASMA_UNITS_STRONG)
+    k = cfg.kernel_constants()
+    assert k["ke"] == pytest.approx(1.0e-3, rel=0.01) and k["kmu"] == pytest.approx(2.8e-4, rel=0.01)
+    assert k["kkge"] == pytest.approx(-2.50, rel=0.01)
