@@ -191,6 +191,54 @@ __device__ __forceinline__ void store_received(typename S::T* __restrict__ buf, 
     });
 }
 
+// 2-byte codes: store_ddfs with the +x neighbour stores' codes captured in
+// cap[slot] instead of stored (they are packed in pairs into 32-bit LDS words
+// once all three lattices are done), and the stores of captured codes
+// (store_cap: the cell's own targets; store_received_codes: the targets of the
+// cell one back along x, one cell back from this cell's neighbours)
+template <int Q, int NQ, class S>
+__device__ __forceinline__ void store_ddfs_cap(typename S::T* __restrict__ buf, uint64_t N, uint32_t n, const Nbr& b,
+                                               uint32_t odd, const typename S::R* fh, typename S::T* cap) {
+    *at(buf, n) = S::st(fh[0]);
+    static_for<(NQ - 1) / 2>([&](auto p) {
+        constexpr int i = 2 * decltype(p)::value + 1;
+        const uint32_t j = nb_index<Q, i>(b);
+        typename S::T* so = buf + (uint64_t)(i + 1u - odd) * N;
+        typename S::T* sn = buf + (uint64_t)(i + odd) * N;
+        constexpr int ks = xslot<Q, i>();
+        if constexpr (ks >= 0) cap[ks] = S::st(fh[i]);
+        else __builtin_nontemporal_store(S::st(fh[i]), at(sn, j));
+        __builtin_nontemporal_store(S::st(fh[i + 1]), at(so, n));
+    });
+}
+template <int Q, int NQ, class S>
+__device__ __forceinline__ void store_cap(typename S::T* __restrict__ buf, uint64_t N, const Nbr& b, uint32_t odd,
+                                          const typename S::T* cap) {
+    static_for<(NQ - 1) / 2>([&](auto p) {
+        constexpr int i = 2 * decltype(p)::value + 1;
+        constexpr int ks = xslot<Q, i>();
+        if constexpr (ks >= 0) {
+            typename S::T* sn = buf + (uint64_t)(i + odd) * N;
+            __builtin_nontemporal_store(cap[ks], at(sn, nb_index<Q, i>(b)));
+        }
+    });
+}
+template <int Q, int NQ, class S>
+__device__ __forceinline__ void store_received_codes(typename S::T* __restrict__ buf, uint64_t N, const Nbr& b,
+                                                     uint32_t odd, const typename S::T* code) {
+    static_for<(NQ - 1) / 2>([&](auto p) {
+        constexpr int i = 2 * decltype(p)::value + 1;
+        constexpr int ks = xslot<Q, i>();
+        if constexpr (ks >= 0) {
+            constexpr int cy = VS<Q>::cy[i], cz = VS<Q>::cz[i];
+            const uint32_t y = cy == 0 ? b.y0 : (cy > 0 ? b.yp : b.ym);
+            const uint32_t z = cz == 0 ? b.z0 : (cz > 0 ? b.zp : b.zm);
+            typename S::T* sn = buf + (uint64_t)(i + odd) * N;
+            __builtin_nontemporal_store(code[ks], at(sn, b.x0 + y + z));
+        }
+    });
+}
+
 // ---------------------------------------------------------------------------
 // initialize, sim_kernels.cl:760-832 (writes the t = 1 layout)
 // ---------------------------------------------------------------------------
@@ -288,17 +336,26 @@ __global__ void __launch_bounds__(MHD ? 512 : 256)
     active = active && bo != 0x01;
     V4 lod = mk4<R>(R(0), R(0), R(0), R(0));
     typedef typename S::T T;
-    // +x neighbour stores through LDS (store_ddfs_x): the D3Q19 MHD variants with
-    // the fused LOD tree (whose barrier they share), 4-byte storage. With 2-byte
-    // codes the hand-over measured 2 % slower than the straddling stores
-    // (FP16C 256^3: 0.854 against 0.838 ms), so those variants store directly.
-    // LDS after the LOD partials: the deposited values [NXS][W], then one
-    // fluid flag per work-item.
+    // +x neighbour stores through LDS: the D3Q19 MHD variants with the fused LOD
+    // tree (whose barrier they share). 4-byte storage (store_ddfs_x): the values
+    // [NXS][W] are deposited as each lattice is stored. 2-byte codes: one 16-bit
+    // LDS write per code measured 2 % slower than the straddling stores (FP16C
+    // 256^3: 0.854 against 0.838 ms), so the 11 codes are held in registers
+    // until the last lattice is stored and handed over as 6 words of two codes
+    // ([6][W]; FP16C 0.833 -> 0.810 ms, FP16S 0.824 -> 0.755 ms, bit-identical).
+    // Then one fluid flag per work-item.
     constexpr bool XCH = MHD && LOD && !ECR && Q == 19 && sizeof(T) == 4;
+    constexpr bool XCH16 = MHD && LOD && !ECR && Q == 19 && sizeof(T) == 2;
     constexpr int NXS = XCH ? 2 * xslots<19, 19>() + xslots<19, 7>() : 1;
     const uint32_t BW = blockDim.x;
     T* const xch = (T*)(lds_raw + (size_t)BW * sizeof(V4));
-    uint8_t* const xfl = (uint8_t*)(xch + (size_t)NXS * BW);
+    uint32_t* const xch32 = (uint32_t*)(lds_raw + (size_t)BW * sizeof(V4));  // [6][W] code pairs
+    uint8_t* const xfl = XCH16 ? (uint8_t*)(xch32 + 6 * (size_t)BW) : (uint8_t*)(xch + (size_t)NXS * BW);
+    T cap_f[5], cap_e[5], cap_q[1];  // XCH16: the +x stores' codes until the hand-over
+    (void)xch32;
+    (void)cap_f;
+    (void)cap_e;
+    (void)cap_q;
     // work-item of the cell one step along +x when it is in this block
     // (blocks hold whole rows, or whole LOD segments of one row)
     auto xdst = [&](uint32_t x) -> uint32_t {
@@ -422,6 +479,7 @@ __global__ void __launch_bounds__(MHD ? 512 : 256)
 #pragma unroll
             for (int i = 0; i < 7; i++) qhn[i] = inlet ? qeq[i] : fmaR(R(1) - wq, qhn[i], wq * qeq[i]);
             if constexpr (XCH) store_ddfs_x<19, 7, S, 2 * xslots<19, 19>()>((T*)a.fqi, a.N_store, n, nb_store(), a.odd, qhn, xch, BW, xdst(b.x0));
+            else if constexpr (XCH16) store_ddfs_cap<19, 7, S>((T*)a.fqi, a.N_store, n, nb_store(), a.odd, qhn, cap_q);
             else store_ddfs<19, 7, S>((T*)a.fqi, a.N_store, n, nb_store(), a.odd, qhn);
             // electron gas part 2: F = -rho_e (E + u_e x B), Guo forcing, SRT
             const R crx = uyn_e * Bz - uzn_e * By;
@@ -440,6 +498,7 @@ __global__ void __launch_bounds__(MHD ? 512 : 256)
                 ehn[i] = is_e ? feq[i] : fmaR(R(1) - w, ehn[i], fmaR(w, feq[i], fi));
             }
             if constexpr (XCH) store_ddfs_x<Q, Q, S, xslots<19, 19>()>((T*)a.ei, a.N_store, n, nb_store(), a.odd, ehn, xch, BW, xdst(b.x0));
+            else if constexpr (XCH16) store_ddfs_cap<Q, Q, S>((T*)a.ei, a.N_store, n, nb_store(), a.odd, ehn, cap_e);
             else store_ddfs<Q, Q, S>((T*)a.ei, a.N_store, n, nb_store(), a.odd, ehn);
             // EM force on the gas (plain multiply-add, as written)
             fxn += rhon_q * (Ex + uyn * Bz - uzn * By);
@@ -506,6 +565,24 @@ __global__ void __launch_bounds__(MHD ? 512 : 256)
                                                fmaR(R(0.5) * wm, feq[i] - feb[i] - fhn[i] + fhb[i], fhn[i] + Fin[i]));
         }
         if constexpr (XCH) store_ddfs_x<Q, Q, S, 0>((T*)a.fi, a.N_store, n, nb_store(), a.odd, fhn, xch, BW, xdst(b.x0));
+        else if constexpr (XCH16) {
+            store_ddfs_cap<Q, Q, S>((T*)a.fi, a.N_store, n, nb_store(), a.odd, fhn, cap_f);
+            const uint32_t dst = xdst(b.x0);
+            if (dst != XCH_NONE) {  // to the work-item of cell n + 1 along x, two codes per word
+                auto pk = [](uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); };
+                xch32[0 * BW + dst] = pk(cap_f[0], cap_f[1]);
+                xch32[1 * BW + dst] = pk(cap_f[2], cap_f[3]);
+                xch32[2 * BW + dst] = pk(cap_f[4], cap_e[0]);
+                xch32[3 * BW + dst] = pk(cap_e[1], cap_e[2]);
+                xch32[4 * BW + dst] = pk(cap_e[3], cap_e[4]);
+                xch32[5 * BW + dst] = (uint32_t)cap_q[0];
+            } else {  // the target leaves the block: store directly
+                const Nbr bs = nb_store();
+                store_cap<Q, Q, S>((T*)a.fi, a.N_store, bs, a.odd, cap_f);
+                store_cap<Q, Q, S>((T*)a.ei, a.N_store, bs, a.odd, cap_e);
+                store_cap<19, 7, S>((T*)a.fqi, a.N_store, bs, a.odd, cap_q);
+            }
+        }
         else store_ddfs<Q, Q, S>((T*)a.fi, a.N_store, n, nb_store(), a.odd, fhn);
     }
 
@@ -513,8 +590,26 @@ __global__ void __launch_bounds__(MHD ? 512 : 256)
         // level-1 pairwise tree over each LOD row segment
         const uint32_t bx = a.lod_bx;
         lds_lod[tid] = lod;
-        if constexpr (XCH) xfl[tid] = active ? 1 : 0;
+        if constexpr (XCH || XCH16) xfl[tid] = active ? 1 : 0;
         __syncthreads();
+        if constexpr (XCH16) {
+            if (n64 < N) {
+                const Nbr br = neighbours(n, a.nx, a.ny, a.nz);
+                const uint32_t NX = a.nx;
+                const uint32_t src = NX <= BW ? (br.x0 == 0u ? tid + NX - 1u : tid - 1u) : (tid > 0u ? tid - 1u : XCH_NONE);
+                if (src != XCH_NONE && xfl[src]) {
+                    uint32_t w[6];
+#pragma unroll
+                    for (int k = 0; k < 6; k++) w[k] = xch32[k * BW + tid];
+                    const T rf[5] = {(T)w[0], (T)(w[0] >> 16), (T)w[1], (T)(w[1] >> 16), (T)w[2]};
+                    const T re[5] = {(T)(w[2] >> 16), (T)w[3], (T)(w[3] >> 16), (T)w[4], (T)(w[4] >> 16)};
+                    const T rq[1] = {(T)w[5]};
+                    store_received_codes<Q, Q, S>((T*)a.fi, a.N_store, br, a.odd, rf);
+                    store_received_codes<Q, Q, S>((T*)a.ei, a.N_store, br, a.odd, re);
+                    store_received_codes<19, 7, S>((T*)a.fqi, a.N_store, br, a.odd, rq);
+                }
+            }
+        }
         if constexpr (XCH) {
             // the +x neighbour stores deposited for this cell by the cell one back
             // along x (if it is in the block and not solid)
@@ -666,10 +761,8 @@ static hipError_t sc_impl(bool mhd, bool trt, bool lod, uint32_t block, const Lb
     const uint32_t g = blocks_for(a.N, block);
     // LOD partials, and for the D3Q19 MHD variants the +x stores handed over through LDS
     // (11 values: 5 of fi, 5 of ei, 1 of fqi) plus one fluid flag per work-item
-    constexpr bool xch = Q == 19 && sizeof(typename S::T) == 4;
-    const size_t sh = lod ? (size_t)block * (sizeof(typename V4<typename S::R>::T) +
-                                            (mhd && xch ? 11 * sizeof(typename S::T) + 1 : 0))
-                          : 0;
+    constexpr size_t xch = Q != 19 ? 0 : (sizeof(typename S::T) == 4 ? 11 * 4 + 1 : 6 * 4 + 1);
+    const size_t sh = lod ? (size_t)block * (sizeof(typename V4<typename S::R>::T) + (mhd ? xch : 0)) : 0;
 #define ION_SC(M, T, L) hipLaunchKernelGGL((k_stream_collide<Q, S, M, T, L>), dim3(g), dim3(block), sh, s, a)
     if (mhd) {
         if (lod) { if (trt) ION_SC(true, true, true); else ION_SC(true, false, true); }
