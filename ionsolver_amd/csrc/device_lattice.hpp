@@ -182,8 +182,14 @@ __device__ __forceinline__ uint16_t fp16c_encode(float x) {
     // A NaN of either sign is encoded as the positive quiet NaN (0/0 on the GPU).
     const uint32_t b = (x != x ? 0x7FC00000u : __float_as_uint(x)) + 0x00000800u;
     const uint32_t nrm = (b >> 12) & 0x7FFFu;
-    const uint32_t den = ((uint32_t)(__builtin_fabsf(x) * 0x1p26f) + 1u) >> 1;
-    const uint32_t mag = (b & 0x7FFFFFFFu) >= 0x38800000u ? nrm : den;
+    // floor(|x| 2^25 + 1/2): v_cvt_rpi_i32_f32 converts with round-half-up and
+    // no intermediate addition (checked on the GPU for every float in [0, 4096):
+    // equal to (u32(2y) + 1) >> 1), 2 operations instead of 4
+    int den;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(den) : "v"(__builtin_fabsf(x) * 0x1p25f));
+    // (b & 0x7FFFFFFF) >= 0x38800000 <=> |x| >= 0x387FF800 as a float, or x is NaN
+    const bool big = !(__builtin_fabsf(x) < __uint_as_float(0x387FF800u));
+    const uint32_t mag = big ? nrm : (uint32_t)den;
     return (uint16_t)(((b >> 16) & 0x8000u) | mag);
 }
 __device__ __forceinline__ float fp16c_decode(uint16_t x) {

@@ -141,6 +141,16 @@ def test_device_codecs_match_oracle():
     g, o = I.device_codec(1, 0, sweep), O.fp16c_encode(sweep)
     bad = np.flatnonzero(g != o)
     assert bad.size == 0, (bad.size, sweep[bad[:8]].view(np.uint32), g[bad[:8]], o[bad[:8]])
+    # the denormal codes' rounding steps: the encoder rounds |x| 2^25 half up, so
+    # the code changes at x = (k + 1/2) 2^-25; the float below, at and above every
+    # such point, k = 0 .. 2047, both signs
+    k = np.arange(2048, dtype=np.float64)
+    mid = ((k + 0.5) * 2.0 ** -25).astype(np.float32)
+    steps = np.concatenate([np.nextafter(mid, np.float32(0)), mid, np.nextafter(mid, np.float32(1))])
+    steps = np.concatenate([steps, -steps])
+    g, o = I.device_codec(1, 0, steps), O.fp16c_encode(steps)
+    bad = np.flatnonzero(g != o)
+    assert bad.size == 0, (bad.size, steps[bad[:8]], g[bad[:8]], o[bad[:8]])
     for ft, dec in ((1, O.fp16c_decode), (0, O.fp16s_load)):
         g, o = I.device_codec(ft, 1, codes), dec(codes)
         assert np.array_equal(g.view(np.uint32)[~np.isnan(o)], o.view(np.uint32)[~np.isnan(o)])
