@@ -16,6 +16,8 @@
 
 #include "surrogate.hpp"
 
+#include <algorithm>
+
 namespace ion {
 namespace sur {
 
@@ -65,6 +67,10 @@ constexpr int BM = 64, BN = 64, BK = 32, LDS_PAD = 8;
 template <int EPI>
 __device__ __forceinline__ void epilogue(const GemmOut& out, int M, int N, int m, int n, float acc) {
     if (m >= M || n >= N) return;
+    if (EPI == EPI_PARTIAL) {  // split K: this block's partial sum, reduced by k_splitk_reduce
+        out.f32[((size_t)blockIdx.z * M + m) * N + n] = acc;
+        return;
+    }
     const float v = out.alpha * acc;
     if (EPI == EPI_F32) {
         float* p = out.f32 + (size_t)m * out.ldc + n;
@@ -109,7 +115,10 @@ __global__ void __launch_bounds__(256) k_gemm_nt(const __bf16* __restrict__ A, c
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
     const int lr = tid >> 2, lk = (tid & 3) * 8;  // loader: row, k offset (8 bf16 = 16 bytes)
-    for (int k0 = 0; k0 < K; k0 += BK) {
+    // split K: block z takes the z-th of gridDim.z equal runs of K slices
+    const int nk = K / BK, per = (nk + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int kb = (int)blockIdx.z * per * BK, ke = min(K, kb + per * BK);
+    for (int k0 = kb; k0 < ke; k0 += BK) {
         bf16x8 va = (bf16x8){}, vb = (bf16x8){};
         if (m0 + lr < M) va = *(const bf16x8*)(A + (size_t)(m0 + lr) * lda + k0 + lk);
         if (n0 + lr < N) vb = *(const bf16x8*)(B + (size_t)(n0 + lr) * ldb + k0 + lk);
@@ -696,6 +705,17 @@ static inline dim3 grid_for(size_t n) {
     return dim3((uint32_t)b);
 }
 
+// Sum of the split-K partials [splits][M][N] in split order, then the epilogue.
+template <int EPI>
+__global__ void __launch_bounds__(256) k_splitk_reduce(GemmOut out, int M, int N, int splits) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M * N) return;
+    const float* ws = out.splitk_ws;
+    float acc = ws[idx];
+    for (int z = 1; z < splits; z++) acc += ws[(size_t)z * M * N + idx];
+    epilogue<EPI>(out, M, N, idx / N, idx % N, acc);
+}
+
 hipError_t gemm_nt(int epi, const __bf16* A, const __bf16* B, int M, int N, int K, int lda, int ldb, const GemmOut& o,
                    hipStream_t s) {
     if (K % BK != 0 || M <= 0 || N <= 0) return hipErrorInvalidValue;
@@ -709,6 +729,27 @@ hipError_t gemm_nt(int epi, const __bf16* A, const __bf16* B, int M, int N, int 
         return hipGetLastError();
     }
     const dim3 g((N + BN - 1) / BN, (M + BM - 1) / BM);
+    // Few tiles and a long K (the U-Net's deep layers: 1-64 tiles, 7-95 K
+    // slices): split K so that at least ~256 blocks run, each over a run of
+    // K slices, into fp32 partials that a second kernel sums in a fixed order.
+    const int nk = K / BK, tiles = (int)(g.x * g.y);
+    int splits = 1;
+    if (o.splitk_ws != nullptr && (epi == EPI_F32 || epi == EPI_BF16) && tiles < 256 && nk >= 4) {
+        splits = std::min(std::min(nk / 2, 32), (256 + tiles - 1) / tiles);
+        while (splits > 1 && (size_t)splits * M * N > o.splitk_cap) splits--;
+        const int per = (nk + splits - 1) / splits;
+        splits = (nk + per - 1) / per;  // no empty run
+    }
+    if (splits > 1) {
+        GemmOut po = o;
+        po.f32 = o.splitk_ws;
+        hipLaunchKernelGGL(k_gemm_nt<EPI_PARTIAL>, dim3(g.x, g.y, (uint32_t)splits), dim3(256), 0, s, A, B, M, N, K, lda,
+                           ldb, po);
+        const dim3 gr((uint32_t)((M * N + 255) / 256));
+        if (epi == EPI_F32) hipLaunchKernelGGL(k_splitk_reduce<EPI_F32>, gr, dim3(256), 0, s, o, M, N, splits);
+        else hipLaunchKernelGGL(k_splitk_reduce<EPI_BF16>, gr, dim3(256), 0, s, o, M, N, splits);
+        return hipGetLastError();
+    }
     if (epi == EPI_F32) hipLaunchKernelGGL(k_gemm_nt<EPI_F32>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
     else if (epi == EPI_BF16) hipLaunchKernelGGL(k_gemm_nt<EPI_BF16>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);
     else hipLaunchKernelGGL(k_gemm_nt<EPI_HEAD>, g, dim3(256), 0, s, A, B, M, N, K, lda, ldb, o);

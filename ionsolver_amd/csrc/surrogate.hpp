@@ -7,7 +7,7 @@
 namespace ion {
 namespace sur {
 
-enum { EPI_F32 = 0, EPI_BF16 = 1, EPI_HEAD = 2 };
+enum { EPI_F32 = 0, EPI_BF16 = 1, EPI_HEAD = 2, EPI_PARTIAL = 3 };
 
 // Output of a GEMM: fp32 / bf16 matrix with leading dimension ldc, or the
 // head epilogue that scatters block x (component, offset) onto the cells.
@@ -31,6 +31,11 @@ struct GemmOut {
     int raw_out;              // write the raw source sums of the fluid cells into e_dyn / b_dyn (for the judge)
     uint64_t ncell;
     uint32_t nx, ny, nz, nd, bx, by, bz, cpb;
+    // split K (64 x 64 tiles, EPI_F32 / EPI_BF16): fp32 partial sums
+    // [splits][M][N] of up to splitk_cap floats, summed in split order by a
+    // second kernel that applies the epilogue (null: no split)
+    float* splitk_ws;
+    size_t splitk_cap;
 };
 
 struct HeadGeom {

@@ -142,6 +142,7 @@ int Net::create(const Geom& geo, const Config& cfg, hipStream_t stream) {
         (rc = add_layer(C[1], C[0], C[0], g0, &layers_[D0])))
         return rc;
     // pooled / upsampled activations and their gradients
+    if ((rc = alloc((void**)&splitk_ws_, SPLITK_CAP * sizeof(float)))) return rc;
     if ((rc = alloc((void**)&x_, (size_t)g0 * g0 * g0 * 8 * 2)) || (rc = alloc((void**)&scale_, 256)) ||
         (rc = alloc((void**)&far_, (size_t)g0 * g0 * g0 * FAR_PTS * 6 * 4)))
         return rc;
@@ -245,6 +246,8 @@ int Net::conv_fwd(Layer& L, const __bf16* a, const __bf16* b) {
     o.bf16 = L.out;
     o.ldc = L.cout;
     o.alpha = 1.0f;
+    o.splitk_ws = splitk_ws_;
+    o.splitk_cap = SPLITK_CAP;
     SCHK(gemm_nt(EPI_BF16, L.col, L.w, L.vox, L.cout, L.kpad, L.kpad, L.kpad, o, stream_));
     return 0;
 }

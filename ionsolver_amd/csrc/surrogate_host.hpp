@@ -97,6 +97,9 @@ class Net {
     Layer layers_[NLAYER];
     __bf16* x_ = nullptr;
     float* far_ = nullptr;  // far-field samples [block][6][27]
+    // split-K partial sums of the U-Net's forward GEMMs (few output tiles, long K)
+    static constexpr size_t SPLITK_CAP = (size_t)1 << 20;
+    float* splitk_ws_ = nullptr;
     float* scale_ = nullptr;
     __bf16* pool_[3] = {nullptr, nullptr, nullptr};
     float* dpool_[3] = {nullptr, nullptr, nullptr};
