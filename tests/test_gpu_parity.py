@@ -64,6 +64,20 @@ def test_mhd_rows_wider_than_a_block(nx, ny, nz, D, ft):
     assert compare_all(gpu, orc, info) == []
 
 
+@pytest.mark.parametrize("nx,ny,nz,D,ft", [(24, 12, 12, 2, 2), (24, 12, 12, 2, 1), (24, 12, 12, 2, 0),
+                                           (40, 20, 10, 1, 1), (96, 6, 6, 1, 2), (300, 4, 4, 2, 1)])
+def test_mhd_rows_and_lod_blocks_not_powers_of_two(nx, ny, nz, D, ft):
+    """Row lengths and LOD block sides that are not powers of two (as the
+    384^3 workload's 24-cell blocks): stream_collide blocks of 216, 200, 192
+    and 150 work-items (the last wavefront partial), the periodic wrap of the
+    +x hand-over inside whole rows, and LOD segments of 6, 20, 48 and 75
+    cells summed by the pairwise tree. Every buffer exact."""
+    rng = np.random.default_rng(700 + nx + ny + D + ft)
+    gpu, orc, info = make_pair(rng, nx=nx, ny=ny, nz=nz, mhd=True, lod_depth=D, float_type=ft)
+    run_both(gpu, orc, 3)
+    assert compare_all(gpu, orc, info) == []
+
+
 @pytest.mark.parametrize("shape,D", [((64, 64, 64), 3), ((128, 64, 64), 3), ((192, 64, 64), 3), ((32, 64, 64), 3),
                                      ((64, 64, 32), 2), ((96, 64, 64), 2)])
 def test_exact_field_on_lod_tiles(shape, D):
