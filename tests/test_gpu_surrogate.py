@@ -22,16 +22,18 @@ from ionsolver_amd.surrogate import evaluate, lod_buffer, train
 pytestmark = pytest.mark.gpu
 
 
-def make_lbm(n=16, D=3, solid_frac=0.0, seed=0, channels=8, **kw):
-    cfg = make_config(nx=n, ny=n, nz=n, mhd=True, lod_depth=D, graphics=False)
+def make_lbm(n=16, D=3, solid_frac=0.0, seed=0, channels=8, shape=None, **kw):
+    nx, ny, nz = shape or (n, n, n)
+    cells = nx * ny * nz
+    cfg = make_config(nx=nx, ny=ny, nz=nz, mhd=True, lod_depth=D, graphics=False)
     lbm = I.Lbm(cfg)
     if solid_frac > 0:
         rng = np.random.default_rng(seed)
-        flags = np.where(rng.uniform(size=n ** 3) < solid_frac, 1, 0).astype(np.uint8)
+        flags = np.where(rng.uniform(size=cells) < solid_frac, 1, 0).astype(np.uint8)
         lbm.write_field(Field.FLAGS, flags)
     rng = np.random.default_rng(seed + 1)
-    lbm.write_field(Field.E_STAT, (0.01 * rng.standard_normal(3 * n ** 3)).astype(np.float32))
-    lbm.write_field(Field.B_STAT, (0.01 * rng.standard_normal(3 * n ** 3)).astype(np.float32))
+    lbm.write_field(Field.E_STAT, (0.01 * rng.standard_normal(3 * cells)).astype(np.float32))
+    lbm.write_field(Field.B_STAT, (0.01 * rng.standard_normal(3 * cells)).astype(np.float32))
     lbm.surrogate_create(channels=channels, **kw)
     return lbm
 
@@ -72,19 +74,23 @@ def test_initial_head_is_the_direct_near_field():
     assert rel_l2(b_sur, b_dir) < 1e-2
 
 
-@pytest.mark.parametrize("n,solid_frac", [(16, 0.0), (16, 0.2), (32, 0.1), (64, 0.1)])
-def test_forward_loss_and_gradients_vs_float64_reference(n, solid_frac):
+@pytest.mark.parametrize("shape,solid_frac", [((16, 16, 16), 0.0), ((16, 16, 16), 0.2), ((32, 32, 32), 0.1),
+                                              ((64, 64, 64), 0.1), ((32, 16, 16), 0.1), ((128, 64, 32), 0.1)])
+def test_forward_loss_and_gradients_vs_float64_reference(shape, solid_frac):
     """16^3: every GEMM on the 64 x 64 kernel; 32^3: the head and the large
     input-gradient GEMMs on the 128 x 128 kernel, N tiles innermost; 64^3:
     the head with more N tiles (24) than M tiles (4), in the XCD-grouped
-    order whose M group of 8 tiles is half padding."""
-    lbm = make_lbm(n, 3, solid_frac=solid_frac, seed=3)
-    geo = S.Geometry(n, n, n, 3, 8)
+    order whose M group of 8 tiles is half padding. Non-cubic lattices, whose
+    LOD blocks have three different sides (the head's cell addressing and the
+    per-axis far-field interpolation): 32 x 16 x 16 (4 x 2 x 2 blocks, the
+    64 x 64 head) and 128 x 64 x 32 (16 x 8 x 4 blocks, the 128 x 128 head)."""
+    lbm = make_lbm(D=3, solid_frac=solid_frac, seed=3, shape=shape)
+    geo = S.Geometry(*shape, 3, 8)
     rng = np.random.default_rng(4)
     params = np.concatenate([rng.standard_normal(r * c) / np.sqrt(27 * 8) for r, c in geo.shapes]).astype(np.float32)
     lbm.set_surrogate_params(params)
     np.testing.assert_array_equal(lbm.surrogate_params(), params)
-    top = random_lod((n, n, n), 3, rng)
+    top = random_lod(shape, 3, rng)
     lbm.write_field(Field.QU_LOD, lod_buffer(lbm, top))
     lbm.compute_fields()
     e_dyn = lbm.read_field(Field.E_DYN)
