@@ -625,17 +625,31 @@ __global__ void __launch_bounds__(MHD ? 512 : 256)
             }
         }
         const uint32_t i = tid % bx;
-        uint32_t P = 1;
-        while (P < bx) P <<= 1;
-        for (uint32_t s = P >> 1; s >= 1; s >>= 1) {
-            if (i < s) {
-                const V4 v = lds_lod[tid];
-                const V4 o = (i + s < bx) ? lds_lod[tid + s] : mk4<R>(R(0), R(0), R(0), R(0));
-                lds_lod[tid] = mk4<R>(v.x + o.x, v.y + o.y, v.z + o.z, v.w + o.w);
+        if (bx <= 64u && (bx & (bx - 1u)) == 0u) {
+            // segments of a power of two up to 64 cells lie inside one
+            // wavefront: the same pairwise tree (same pairs, same order, so
+            // the same sums) with cross-lane reads instead of LDS and four
+            // barriers (FP16C 256^3: 0.794 -> 0.780 ms, FP16S 0.758 -> 0.751)
+            V4 v = lod;
+            for (uint32_t s = bx >> 1; s >= 1; s >>= 1) {
+                const V4 o = mk4<R>(__shfl_down(v.x, s), __shfl_down(v.y, s), __shfl_down(v.z, s), __shfl_down(v.w, s));
+                if (i < s) v = mk4<R>(v.x + o.x, v.y + o.y, v.z + o.z, v.w + o.w);
             }
-            __syncthreads();
+            if (i == 0 && n64 < N) Lseg[n64 / bx] = v;
+        } else {
+            uint32_t P = 1;
+            while (P < bx) P <<= 1;
+            for (uint32_t s = P >> 1; s >= 1; s >>= 1) {
+                if (i < s) {
+                    const V4 v = lds_lod[tid];
+                    const V4 o = (i + s < bx) ? lds_lod[tid + s] : mk4<R>(R(0), R(0), R(0), R(0));
+                    lds_lod[tid] = mk4<R>(v.x + o.x, v.y + o.y, v.z + o.z, v.w + o.w);
+                }
+                __syncthreads();
+            }
+            if (i == 0 && n64 < N) Lseg[n64 / bx] = lds_lod[tid];
         }
-        if (i == 0 && n64 < N) Lseg[n64 / bx] = lds_lod[tid];
+
     }
 }
 
