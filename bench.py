@@ -41,7 +41,7 @@ FLOP_PER_INTERACTION = 35  # r 3, r^2 5, sqrt 1, cube 2, r/|r|^3 3, e 6, v x pre
 # (FETCH_SIZE x 2, WRITE_SIZE x 1). Kernels compiled since may differ.
 PMC_TRAFFIC = {
     ("mhd256", "stream_collide"): 6.493e9,
-    ("mhd256-fp16c", "stream_collide"): 3.687e9,
+    ("mhd256-fp16c", "stream_collide"): 3.688e9,
     ("mhd256", "update_e_b_dynamic"): 1.438e9,
     ("mhd256-fp16c", "update_e_b_dynamic"): 1.438e9,
 }
