@@ -163,7 +163,12 @@ extern "C" int ion_load(const char* path, ion_config* cfg, ion_handle** out) {
     c.ext_magneto_hydro = (ext & 8) != 0;
     c.mhd_lod_depth = r.u8();
     if (!r.ok) return ion_internal_set_error(ION_ERR_FORMAT, "truncated header");
-    const uint64_t N = (uint64_t)c.n_x * c.n_y * c.n_z;
+    // the section lengths follow from the lattice: bounded first (n_x * n_y
+    // before n_z, so that the product cannot wrap), as ion_create bounds it
+    const uint64_t nxy = (uint64_t)c.n_x * c.n_y;
+    if (nxy > 0x40000000ull || nxy * c.n_z > 0x40000000ull)
+        return ion_internal_set_error(ION_ERR_UNSUPPORTED, "lattice of more than 2^30 cells in the header");
+    const uint64_t N = nxy * c.n_z;
     const bool mhd = c.ext_magneto_hydro != 0;
     const uint8_t* flags = r.take(N);
     const uint8_t* rho = r.take(4 * N);

@@ -415,10 +415,12 @@ int validate(const ion_config& c) {
     if (c.n_x == 0 || c.n_y == 0 || c.n_z == 0) return fail(ION_ERR_INVALID, "lattice size must be positive");
     if (c.d_x != 1 || c.d_y != 1 || c.d_z != 1)
         return fail(ION_ERR_UNSUPPORTED, "multi-domain split (d_x, d_y, d_z != 1) is replaced by a single GPU");
-    const uint64_t N = (uint64_t)c.n_x * c.n_y * c.n_z;
-    if (N > 0x40000000ull)
+    // n_x * n_y first, so that the product of three 32-bit sides cannot wrap
+    const uint64_t nxy = (uint64_t)c.n_x * c.n_y;
+    if (nxy > 0x40000000ull || nxy * c.n_z > 0x40000000ull)
         return fail(ION_ERR_UNSUPPORTED, "more than 2^30 cells (1024^3) on one GPU: the kernels address every "
                                          "per-direction array with 32-bit byte offsets");
+    const uint64_t N = nxy * c.n_z;
     if (c.float_type == ION_FP64 && N > 0x20000000ull)
         return fail(ION_ERR_UNSUPPORTED, "more than 2^29 cells with FP64: the kernels address every per-direction "
                                          "array with 32-bit byte offsets, and 8-byte elements reach 2^32 bytes there");

@@ -105,6 +105,8 @@ def base_config(**kw):
     (dict(ext_force_field=True), -1),
     (dict(n_x=1024, n_y=1024, n_z=1025, float_type=I.FloatType.FP32), -2),  # 2^30 + 2^20 cells
     (dict(n_x=1024, n_y=1024, n_z=513, float_type=I.FloatType.FP64), -2),  # FP64 above 2^29 cells
+    (dict(n_x=1 << 22, n_y=1 << 22, n_z=1 << 22), -2),  # 2^66 cells, 4 after a 64-bit wrap
+    (dict(n_x=0xFFFFFFFF, n_y=0xFFFFFFFF, n_z=2), -2),
 ])
 def test_invalid_configs_are_rejected_before_touching_the_device(kw, code):
     with pytest.raises(I.IonError) as e:
@@ -121,6 +123,14 @@ def test_load_errors(tmp_path):
         I.Lbm.load(str(p))
     p.write_bytes(b"IonSolver setup\n" + bytes(10))
     with pytest.raises(I.IonError, match=r"\[-5\]"):
+        I.Lbm.load(str(p))
+    # a header whose lattice (2^22 cubed) wraps to 4 cells in 64 bits, followed
+    # by the sections of 4 cells: rejected before any section is read
+    import struct
+    head = (b"IonSolver setup\n" + struct.pack("<3B4f3IB3I4fBB", 2, 0, 2, 1.0, 1.0, 1.0, 1.0, 1 << 22, 1 << 22,
+                                                 1 << 22, 0, 1, 1, 1, 0.1, 0.0, 0.0, 0.0, 0, 0))
+    p.write_bytes(head + bytes(4 * 17))
+    with pytest.raises(I.IonError, match=r"\[-2\].*2\^30"):
         I.Lbm.load(str(p))
 
 
