@@ -96,7 +96,10 @@ def no_alloc(h, cfg, arr, mode):
 def test_no_alloc_field_step_matches_compute_fields(mode):
     h = hip()
     rng = np.random.default_rng(31)
-    for n, D in ((64, 3), (32, 2), (16, 2)):
+    # depth 5 first (up to 8^5 LOD sources, the static array's capacity; at
+    # 32^3 its near-field block is the whole lattice, as in the reference),
+    # then smaller calls over what the one before left in the array
+    for n, D in ((32, 5), (64, 3), (32, 2), (16, 2)):
         cfg = make_config(nx=n, ny=n, nz=n, mhd=True, lod_depth=D, float_type=2)
         arr = inputs(cfg, rng)
         print(f"no-alloc field step: {n}^3, depth {D}, mode {mode.name}", flush=True)
