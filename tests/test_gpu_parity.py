@@ -78,6 +78,19 @@ def test_mhd_rows_and_lod_blocks_not_powers_of_two(nx, ny, nz, D, ft):
     assert compare_all(gpu, orc, info) == []
 
 
+@pytest.mark.parametrize("ft", [2, 1])
+def test_mhd_lod_depth5_near_field_spans_the_lattice(ft):
+    """LOD depth 5 at 32^3: single-cell LOD blocks (8^5 top-level sources,
+    one-cell LOD segments in stream_collide) and a near-field block side of
+    n_x / 2^(2^5) cells, whose shift OpenCL takes modulo 32 (the block
+    side of sim_kernels.cl:907-938), so the near field spans the whole
+    lattice. Every buffer exact after 2 steps."""
+    rng = np.random.default_rng(800 + ft)
+    gpu, orc, info = make_pair(rng, nx=32, ny=32, nz=32, mhd=True, lod_depth=5, float_type=ft)
+    run_both(gpu, orc, 2)
+    assert compare_all(gpu, orc, info) == []
+
+
 @pytest.mark.parametrize("shape,D", [((64, 64, 64), 3), ((128, 64, 64), 3), ((192, 64, 64), 3), ((32, 64, 64), 3),
                                      ((64, 64, 32), 2), ((96, 64, 64), 2)])
 def test_exact_field_on_lod_tiles(shape, D):
