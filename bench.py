@@ -530,7 +530,7 @@ def field_interactions(cfg, flags, qu_lod):
     N = cfg.n_x * cfg.n_y * cfg.n_z
     z, rem = np.divmod(np.arange(N, dtype=np.int64), cfg.n_x * cfg.n_y)
     y, x = np.divmod(rem, cfg.n_x)
-    own = x // (cfg.n_x // nd) + (y // (cfg.n_y // nd) + z // (cfg.n_z // nd) * nd) * nd
+    own = lo + x // (cfg.n_x // nd) + (y // (cfg.n_y // nd) + z // (cfg.n_z // nd) * nd) * nd
     fl = (flags & 0x1F) != 1
     return int(np.count_nonzero(fl)) * int(live.sum()) - int(np.sum(fl & live[own]))
 
