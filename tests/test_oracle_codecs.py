@@ -42,6 +42,26 @@ def test_fp16s_scale_is_exact_power_of_two():
     assert np.array_equal(O.fp16s_store(vals[ok]), ALL[ok])
 
 
+def test_fp16s_store_is_the_ieee_half_of_x_times_2_to_15():
+    """vstore_half_rte(x * 32768) (domain.rs:773-784) against numpy's IEEE
+    round-to-nearest-even conversion of the same exact product, on values
+    across the range and on the exact midpoints between adjacent codes, where
+    the rounding direction (and any change of the scale) shows."""
+    rng = np.random.default_rng(11)
+    x = (rng.standard_normal(1 << 18) * np.exp2(rng.integers(-40, 2, 1 << 18))).astype(np.float32)
+    codes = rng.integers(0, 0x7BFF, 1 << 16).astype(np.uint16)
+    lo = codes.view(np.float16).astype(np.float64)
+    hi = (codes + 1).astype(np.uint16).view(np.float16).astype(np.float64)
+    mid = ((lo + hi) / 2 / 32768.0).astype(np.float32)  # exact: 12 significant bits
+    assert np.array_equal(mid.astype(np.float64) * 32768.0, (lo + hi) / 2)
+    for v in (x, mid, -mid):
+        with np.errstate(over="ignore"):  # beyond the half range both give the infinity code
+            ref = (v * np.float32(32768.0)).astype(np.float16).view(np.uint16)
+        assert np.array_equal(O.fp16s_store(v), ref)
+    # ties go to the even code
+    assert np.all(O.fp16s_store(mid) % 2 == 0)
+
+
 def test_fp16s_known_answers():
     # domain.rs:773: "+-1.99902344, +-1.86446416E-9, +-1.81898936E-12" (max, smallest
     # normal, smallest denormal). The middle value of the comment is one ulp
