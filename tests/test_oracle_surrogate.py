@@ -118,6 +118,15 @@ def test_output_groups_scale_separately():
     assert s1[0] == s0[0] and s1[1] == s0[1] * 2.0 ** -14
     # network units are the same; raw units follow the scales
     np.testing.assert_allclose(y1, y0, rtol=1e-12, atol=1e-12)
+    # and so do the targets: the raw far field of the slower input has the
+    # same E (E does not see the velocities) and B times 2^-14, and both map
+    # to the same network units
+    raw0 = S.far_interp(geo, S.far_samples(geo, qu))
+    raw1 = S.far_interp(geo, S.far_samples(geo, qv))
+    assert np.array_equal(raw1[:3], raw0[:3])
+    np.testing.assert_allclose(raw1[3:], raw0[3:] * 2.0 ** -14, rtol=1e-12, atol=0)
+    assert np.abs(raw0[3:]).max() > 0
+    np.testing.assert_allclose(S.targets(geo, s1, raw1), S.targets(geo, s0, raw0), rtol=1e-12, atol=0)
 
 
 def _problem(seed, c0=8):
