@@ -131,6 +131,22 @@ def test_voxelize_and_precompute_synthetic(tmp_path):
     assert np.count_nonzero(flags & 0x10) > 400 and np.count_nonzero((flags & 0x1F) == 0x05) > 10
 
 
+def test_voxelize_box_with_an_open_sheet(tmp_path):
+    """A box and an open square sheet in front of it along the rays: flags and
+    B_dyn identical to the oracle's, and the same as for the box alone (the
+    reference's parity rule leaves an open sheet out)."""
+    shape = (16, 16, 16)
+    solid = box((6.3, 3.6, 4.15), (13.7, 10.4, 10.8))
+    sheet = np.array([[[2.5, 3.0, 3.5], [2.5, 11.0, 3.5], [2.5, 11.0, 11.5]],
+                      [[2.5, 3.0, 3.5], [2.5, 11.0, 11.5], [2.5, 3.0, 11.5]]], np.float32)
+    both = write_stl(tmp_path / "bs.stl", np.concatenate([solid, sheet]))
+    alone = write_stl(tmp_path / "b.stl", solid)
+    _, flags_both = run_both(shape, [(both, (0.0, 0.0, 0.0), ModelType.Solid, (0.0, 0.0, 0.0))], precompute=False)
+    _, flags_alone = run_both(shape, [(alone, (0.0, 0.0, 0.0), ModelType.Solid, (0.0, 0.0, 0.0))], precompute=False)
+    assert np.count_nonzero((flags_alone & 0x1F) == 0x01) > 100
+    assert np.array_equal(flags_both, flags_alone)
+
+
 def test_precompute_e_ecr_writes_e_var(tmp_path):
     """precompute_E_ECR (src/lbm/mod.rs:319-331, domain.rs:569-578) runs the
     static_e_from_mesh kernel into E_var instead of E_stat: E_var (zero

@@ -132,6 +132,27 @@ def test_voxelize_box_cell_set(lo, hi):
     assert np.array_equal(flags[inside], (flags0[inside] & 0xE0) | 0x05)
 
 
+def test_voxelize_open_sheet_encloses_nothing():
+    """A box and, in front of it along the rays, an open square sheet: the
+    reference's rule (a cell is inside where the hits ahead and behind the
+    ray's start are both odd, then toggled at each hit, sim_kernels.cl:
+    1150-1231) leaves the sheet out, so the cells are exactly the box's. A
+    rule that took either parity would also fill the cells between the
+    ray's start and the box."""
+    shape = (16, 15, 17)
+    lo, hi = (6.3, 3.6, 4.15), (13.7, 10.4, 10.8)
+    inside, d = box_cells(shape, lo, hi)
+    assert d == 0  # rays along x, from below the sheet at x = 2.5
+    sheet = np.array([[[2.5, 3.0, 3.5], [2.5, 11.0, 3.5], [2.5, 11.0, 11.5]],
+                      [[2.5, 3.0, 3.5], [2.5, 11.0, 11.5], [2.5, 3.0, 11.5]]], np.float32)  # diagonal z = y + 1/2
+    tri = np.concatenate([box(lo, hi), sheet])
+    n = shape[0] * shape[1] * shape[2]
+    flags = np.zeros(n, np.uint8)
+    O.voxelize(shape, flags, None, tri, tri.reshape(-1, 3).min(0), tri.reshape(-1, 3).max(0), 0x01)
+    assert inside.sum() > 0
+    assert np.array_equal(flags == 0x01, inside)
+
+
 @pytest.mark.parametrize("center,r", [((8.2, 8.7, 7.9), 6.3), ((10.4, 7.35, 9.6), 4.9)])
 def test_voxelize_sphere_between_its_balls(center, r):
     shape = (20, 18, 19)
