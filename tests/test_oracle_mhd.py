@@ -327,3 +327,58 @@ def test_electron_density0_starts_the_electron_gas(ft, tol):
     orc.initialize()
     total = orc.ddfs("ei").astype(np.float64).sum() + n ** 3
     assert abs(total - n0 * np.count_nonzero(flags != 1)) < tol * n ** 3
+
+
+@pytest.mark.parametrize("solid", [True, False])
+def test_initialize_solid_cells_rest_and_fields_start_static(solid):
+    """initialize (sim_kernels.cl:760-832) on a uniform lattice given a
+    velocity and a charge: a solid lattice stores the gas equilibrium at rest
+    (momentum 0) and the D3Q7 equilibrium of zero charge, and zeroes u and Q;
+    a fluid one keeps rho u and Q. Either way E_dyn and B_dyn start as E_stat
+    and B_stat."""
+    from test_oracle_lorentz import moments, to_t1_layout
+    n = 4
+    N = n ** 3
+    orc = O.OracleSim(n, n, n, lod_depth=1, ext=MHD_EXT, w=1.2, kkge=-1.0)
+    rho0, u0, q0 = 1.05, np.array([0.03, -0.02, 0.01]), 0.125
+    orc.buf("rho")[:] = rho0
+    orc.buf("u")[:] = np.repeat(u0, N)
+    orc.buf("q")[:] = q0
+    if solid:
+        orc.buf("flags")[:] = 1
+    rng = np.random.default_rng(9)
+    e_stat = rng.standard_normal(3 * N).astype(np.float32)
+    b_stat = rng.standard_normal(3 * N).astype(np.float32)
+    orc.buf("e_stat")[:] = e_stat
+    orc.buf("b_stat")[:] = b_stat
+    orc.initialize()
+    f = orc.ddfs("fi").astype(np.float64)
+    assert np.all(f == f[:, :1])
+    rho, j, _ = moments(to_t1_layout(f[:, 0]))
+    assert abs(rho - rho0) < 1e-6
+    np.testing.assert_allclose(j, 0.0 if solid else rho0 * u0, rtol=0, atol=1e-7)
+    qsum = orc.ddfs("fqi").astype(np.float64)[:, 0].sum() + 1.0  # D3Q7 charge moment
+    assert abs(qsum - (0.0 if solid else q0)) < 1e-7
+    assert np.all(orc.buf("q") == (0.0 if solid else np.float32(q0)))
+    assert np.all(orc.buf("u") == (0.0 if solid else np.repeat(u0, N).astype(np.float32)))
+    assert np.array_equal(orc.buf("e_dyn"), e_stat) and np.array_equal(orc.buf("b_dyn"), b_stat)
+
+
+def test_update_fields_clamps_the_velocity():
+    """update_fields (sim_kernels.cl:834-859) recomputes rho and u from the
+    DDFs and clamps each velocity component to the lattice speed of sound:
+    an equilibrium at u = (0.9, -0.7, 0.1) comes back as (c, -c, 0.1)."""
+    n = 4
+    orc = O.OracleSim(n, n, n, lod_depth=1, ext=O.EXT_VOLUME_FORCE | O.EXT_UPDATE_FIELDS, w=1.2)
+    N = n ** 3
+    u0 = np.array([0.9, -0.7, 0.1])
+    orc.buf("rho")[:] = 1.0
+    orc.buf("u")[:] = np.repeat(u0, N)
+    orc.initialize()  # fi = f_eq(1, u0)
+    orc.buf("u")[:] = 0.0
+    orc.kernel("update_fields")
+    c = np.float32(0.57735027)
+    u = orc.buf("u").reshape(3, N)
+    assert np.all(u[0] == c) and np.all(u[1] == -c)
+    np.testing.assert_allclose(u[2], 0.1, rtol=0, atol=1e-6)
+    np.testing.assert_allclose(orc.buf("rho"), 1.0, rtol=0, atol=1e-6)
