@@ -1009,6 +1009,7 @@ int ion_set_field_solver(ion_handle* h, int solver, uint32_t judge_every, double
     if (solver != ION_SOLVER_DIRECT && solver != ION_SOLVER_SURROGATE) return fail(ION_ERR_INVALID, "invalid solver");
     if (solver == ION_SOLVER_SURROGATE && !h->net) return fail(ION_ERR_INVALID, "no surrogate: call ion_surrogate_create first");
     if (judge_every == 0) return fail(ION_ERR_INVALID, "judge_every must be >= 1");
+    if (!(tolerance >= 0.0)) return fail(ION_ERR_INVALID, "tolerance must be >= 0");  // NaN included
     h->solver = solver;
     h->judge_every = judge_every;
     h->tolerance = tolerance;

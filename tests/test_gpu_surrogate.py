@@ -222,6 +222,11 @@ def test_surrogate_rejects_unsupported_configs():
         lbm.surrogate_create(channels=12)
     with pytest.raises(I.IonError):
         lbm.set_field_solver(FieldSolver.SURROGATE)
+    lbm.surrogate_create(channels=8)
+    for every, tol in ((0, 0.01), (50, float("nan")), (50, -0.01)):
+        with pytest.raises(I.IonError, match="judge_every|tolerance"):
+            lbm.set_field_solver(FieldSolver.SURROGATE, judge_every=every, tolerance=tol)
+    lbm.set_field_solver(FieldSolver.SURROGATE, judge_every=50, tolerance=0.0)
 
 
 def test_thruster_runs_on_the_surrogate_with_its_judge():
