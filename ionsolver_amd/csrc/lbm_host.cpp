@@ -831,6 +831,7 @@ int ion_surrogate_create(ion_handle* h, const ion_surrogate_config* cfg) {
     if (h->cfg.n_x / sh > 1 || h->cfg.n_y / sh > 1 || h->cfg.n_z / sh > 1)
         return fail(ION_ERR_UNSUPPORTED, "the surrogate needs an empty near-field block");
     if (cfg->channels < 8 || cfg->channels % 8 != 0) return fail(ION_ERR_INVALID, "channels must be a positive multiple of 8");
+    if (!(cfg->lr >= 0.0f) || !(cfg->lr_head >= 0.0f)) return fail(ION_ERR_INVALID, "learning rates must be >= 0");
     delete h->net;
     h->net = new ion::sur::Net();
     ion::sur::Geom g{h->cfg.n_x, h->cfg.n_y, h->cfg.n_z, h->D, h->N,

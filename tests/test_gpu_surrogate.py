@@ -220,6 +220,8 @@ def test_surrogate_rejects_unsupported_configs():
     lbm = I.Lbm(make_config(nx=16, ny=16, nz=16, mhd=True, lod_depth=3, graphics=False))
     with pytest.raises(I.IonError):
         lbm.surrogate_create(channels=12)
+    with pytest.raises(I.IonError, match="learning rates"):
+        lbm.surrogate_create(channels=8, lr=-1e-4)
     with pytest.raises(I.IonError):
         lbm.set_field_solver(FieldSolver.SURROGATE)
     lbm.surrogate_create(channels=8)
