@@ -1061,6 +1061,8 @@ int ion_device_codec(int float_type, int dir, const void* src, void* dst, uint64
     if (float_type != ION_FP16S && float_type != ION_FP16C) return fail(ION_ERR_INVALID, "codec needs FP16S or FP16C");
     if (dir != 0 && dir != 1) return fail(ION_ERR_INVALID, "dir must be 0 (encode) or 1 (decode)");
     if (n == 0) return ION_OK;
+    if (!src || !dst) return fail(ION_ERR_INVALID, "null buffer");
+    if (n > (1ull << 40)) return fail(ION_ERR_INVALID, "more than 2^40 values");  // keeps n * 4 from wrapping
     const size_t in_b = n * (dir == 0 ? 4 : 2), out_b = n * (dir == 0 ? 2 : 4);
     void *din = nullptr, *dout = nullptr;
     HIPCHK(hipMalloc(&din, in_b));

@@ -144,3 +144,14 @@ def test_no_alloc_field_step_rejects_depth_beyond_its_scratch_array():
     lib = I.load_library()
     assert lib.ion_update_e_b_noalloc(ctypes.byref(cfg._to_c()), buf, 1, None) != 0
     assert "scratch" in lib.ion_last_error().decode()
+
+
+def test_device_codec_rejects_bad_arguments_before_touching_the_device():
+    import ctypes
+    lib = I.load_library()
+    buf = (ctypes.c_uint8 * 64)()
+    assert lib.ion_device_codec(2, 0, buf, buf, 4) != 0 and "FP16S or FP16C" in lib.ion_last_error().decode()
+    assert lib.ion_device_codec(1, 2, buf, buf, 4) != 0 and "dir" in lib.ion_last_error().decode()
+    assert lib.ion_device_codec(1, 0, None, buf, 4) != 0 and "null" in lib.ion_last_error().decode()
+    assert lib.ion_device_codec(1, 0, buf, buf, 1 << 41) != 0 and "2^40" in lib.ion_last_error().decode()
+    assert lib.ion_device_codec(1, 0, None, None, 0) == 0  # nothing to do
