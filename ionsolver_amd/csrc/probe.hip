@@ -155,7 +155,9 @@ extern "C" int ion_hbm_stream_probe(uint32_t elem_bytes, uint32_t streams, uint3
 
 extern "C" int ion_hbm_pull_probe(uint32_t elem_bytes, uint32_t streams, uint32_t nx, uint32_t ny, uint32_t nz,
                                   uint32_t reps, double* best_ms, double* gb_per_s) {
-    const uint64_t N = (uint64_t)nx * ny * nz;
+    // nx * ny first, so that the product of three 32-bit sides cannot wrap
+    const uint64_t nxy = (uint64_t)nx * ny;
+    const uint64_t N = nxy > (1ull << 30) ? 0 : nxy * nz;
     if (!best_ms || !gb_per_s || N == 0 || N > (1ull << 30) || reps == 0 || (elem_bytes != 2 && elem_bytes != 4) ||
         (streams != 19 && streams != 45)) {
         return ion_internal_set_error(ION_ERR_INVALID, "ion_hbm_pull_probe: elem_bytes 2 or 4, streams 19 or 45, "

@@ -20,7 +20,8 @@ def test_copy_stream_and_pull_probes_return_rates_below_the_hbm_peak():
             assert ms > 0 and 0 < gbs < 1.05 * PEAK_GBS
 
 
-@pytest.mark.parametrize("args", [(3, 19, 8, 8, 8), (2, 20, 8, 8, 8), (2, 19, 0, 8, 8), (4, 45, 2048, 1024, 1024)])
+@pytest.mark.parametrize("args", [(3, 19, 8, 8, 8), (2, 20, 8, 8, 8), (2, 19, 0, 8, 8), (4, 45, 2048, 1024, 1024),
+                                  (2, 19, 1 << 22, 1 << 22, 1 << 22)])  # 2^66 cells: 4 after a 64-bit wrap
 def test_pull_probe_rejects_invalid_arguments(args):
     with pytest.raises(I.IonError, match="ion_hbm_pull_probe"):
         I.hbm_pull_probe(*args, reps=1)
